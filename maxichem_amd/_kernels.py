@@ -1,0 +1,97 @@
+"""The HIP kernels' source for one network: the generated header (sizes, LDS layout, stream rounds, settings) and the
+kernel files, compiled at run time with hiprtc for gfx950.
+
+The reference generates network.h from its tables the same way (maxichem/_kernel_header.py:164-232) for NVRTC; the
+macros here are this build's own layout.
+"""
+
+from __future__ import annotations
+
+from pathlib import Path
+
+from . import _constants as C
+from ._tables import (
+    CHUNK,
+    SV_DBL,
+    SV_END,
+    SV_NEG,
+    SV_PIECE,
+    SV_R_BITS,
+    T_DBL,
+    T_DEST,
+    T_END,
+    T_IDX_BITS,
+    T_NEG,
+    Tables,
+)
+
+KERNEL_DIR = Path(__file__).resolve().parent / "kernels"
+KERNEL_FILES = ("common.hip", "call.hip", "guard.hip", "recover.hip")
+ARCH = "gfx950"
+# hiprtc options: gfx950 code; packed FP32 instructions are kept out of all code (a known hazard on these parts)
+OPTIONS = (
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-ffp-contract=fast",
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-packed-fp32-ops",
+)
+GUARD_THREADS = 256
+
+# The method's settings the kernels read as constexprs of the same names.
+_SETTINGS = (
+    "H_MIN", "CALL_END", "STEP_SAFETY", "ERR_FLOOR", "FAC_NONFINITE", "FACACC", "FACREJ", "FACMAX",
+    "FAC_AFTER_REJECT", "SECMIN", "SECANT_MIN_ORDER", "SECANT_MAX_ORDER", "SECANT_TARGET", "PIVTAU",
+    "ELEMENT_PIVOT_MIN", "REC_RETRY_SPLIT", "GMAX", "NASA_SPLIT_T",
+)  # fmt: skip
+_FLAGS = ("FL_FALLOFF", "FL_SECOND_K0", "FL_THREE_BODY", "FL_TROE", "FL_EFF", "FL_VM11", "FL_PHOTO", "FL_INERT",
+          "FL_NF_SHIFT", "FL_NR_SHIFT")  # fmt: skip
+
+
+def header(tab: Tables, cpw: int, drift: float) -> str:
+    """The network header: every size, offset and setting the kernel files read."""
+    lay = tab.layout()
+    nch = len(tab.chunk_words)
+    rounds = [w.shape[0] for w in tab.chunk_words]
+    offsets = [sum(rounds[:c]) for c in range(nch)]
+    ints = {
+        "WAVE": C.WAVE, "NS": tab.ns, "NR": tab.nr, "NRT": tab.nrt, "NJ": tab.nj, "K0": tab.k0, "NT": tab.nt,
+        "NEL": tab.nel, "RCW": C.RATE_ROW, "RROUNDS": -(-tab.nr // C.WAVE), "NDV": tab.ndv, "NPART": tab.npart,
+        "CPW": cpw, "GUARD_THREADS": GUARD_THREADS, "CHUNK": CHUNK, "NCH": nch,
+        "SV_ROUNDS": tab.sv_words.shape[0], "DIAG_ROUNDS": tab.diag_words.shape[0],
+        "LEAD_ROUNDS": tab.lead_words.shape[0], "REC_RETRIES": C.REC_RETRIES,
+        "T_NEG": T_NEG, "T_DBL": T_DBL, "T_END": T_END, "T_DEST": T_DEST, "SV_NEG": SV_NEG, "SV_DBL": SV_DBL,
+        "SV_END": SV_END, "SV_PIECE": SV_PIECE,
+        **{f"L_{k}": v for k, v in lay.items()},
+        **{f"COUNT_{name.upper()}": i for i, name in enumerate(C.COUNT_SLOTS)},
+        **{name: getattr(C, name) for name in _FLAGS},
+    }  # fmt: skip
+    doubles = {
+        **{name: getattr(C, name) for name in _SETTINGS},
+        "TMIN": C.T_MIN, "TMAX": C.T_MAX, "XSUMTOL": C.X_SUM_TOL, "DRIFT": float(drift),
+        "ETOT": float(tab.atoms.sum()), "KCOLL": C.K_COLL, "KBOLTZ": C.KB, "P0BAR": C.P0,
+    }  # fmt: skip
+    lines = [f"// network header: {tab.ns} species, {tab.nr} reactions, tables {tab.digest()}"]
+    lines += [f"constexpr int {k} = {v};" for k, v in ints.items()]
+    lines += [f"constexpr double {k} = {v!r};" for k, v in doubles.items()]
+    lines += [
+        f"constexpr unsigned T_IDX_MASK = {(1 << T_IDX_BITS) - 1}u;",
+        f"constexpr unsigned SV_R_MASK = {(1 << SV_R_BITS) - 1}u;",
+        f"__device__ constexpr int CH_ROUNDS[{nch}] = {{{', '.join(map(str, rounds))}}};",
+        f"__device__ constexpr int CH_OFF[{nch}] = {{{', '.join(map(str, offsets))}}};",
+        f"#define HAS_TROE {int(tab.has_troe)}",
+        f"#define HAS_EFF {int(tab.has_eff)}",
+        "#define DNAN __longlong_as_double(0x7ff8000000000000ll)",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+def source(tab: Tables, cpw: int, drift: float) -> str:
+    """The whole translation unit hiprtc compiles: the header, then the kernel files in order."""
+    parts = [header(tab, cpw, drift)]
+    for name in KERNEL_FILES:
+        parts.append(f"// ---- {name} ----\n" + (KERNEL_DIR / name).read_text())
+    return "\n".join(parts)

@@ -1,0 +1,107 @@
+// The call kernel: a persistent grid of wavefronts, each stepping one cell at a time from an atomic queue through
+// its rates, Rodas3 tries and step control (the reference's call kernel, maxichem/kernels/call.cuh:5-178).
+
+// One try's four stages from the factors of M0 (call.cuh:119-158): K1 = W^-1 F, K2 = W^-1 (F + 4 K1 / h),
+// K3 = W^-1 (F(N + 2 K1) + (K1 - K2) / h), K4 = W^-1 (F(N + 2 K1 + K3) + (K1 - K2 - 8/3 K3) / h), with
+// W^-1 y = M0^-1 y + c0 (1^T M0^-1 y) M0^-1 w by Sherman-Morrison.
+
+extern "C" __global__ void __launch_bounds__(WAVE* CPW)
+    mxc_call(const Net* __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
+             const double* __restrict__ Pa, const double* __restrict__ dta, double* __restrict__ hio,
+             double* __restrict__ xout, int* __restrict__ doneo, double* __restrict__ noko,
+             int* __restrict__ status, int* __restrict__ count, double* __restrict__ tries,
+             const int* __restrict__ order, const double* __restrict__ Jin, int B, double rtol, double atol,
+             int max_steps) {
+  extern __shared__ double lds[];
+  const int lane = (int)(threadIdx.x & (WAVE - 1)), wv = (int)(threadIdx.x / WAVE);
+  double* ws = lds + wv * L_SIZE;
+  const int waves = (int)gridDim.x * CPW;
+  int q = (int)blockIdx.x * CPW + wv;
+  double c[NT];
+  while (q < B) {
+    const int cell = order ? order[q] : q;
+    const double Tc = Ta[cell], dt = dta[cell], nd = Pa[cell] / (KBOLTZ * Tc);
+    const double* xr = x0 + (size_t)cell * NS;
+    const int* perm = opaque(net)->perm;
+    double Nl = lane < K0 ? xr[perm[lane]] : 0.0, Nt = lane < NT ? xr[perm[K0 + lane]] : 0.0;
+    // the screen (common.cuh:244-279): a NaN enters the sum for a negative or non-finite x or J
+    double p = (lane < K0 ? (Nl >= 0.0 ? Nl : DNAN) : 0.0) + (lane < NT ? (Nt >= 0.0 ? Nt : DNAN) : 0.0);
+    double pm = fmax(lane < K0 ? Nl : 0.0, lane < NT ? Nt : 0.0);
+    const double* Jc = NJ ? Jin + (size_t)cell * NJ : nullptr;
+    for (int b = lane; b < NJ; b += WAVE) p += Jc[b] >= 0.0 ? 0.0 * Jc[b] : DNAN;
+    const double ps = wave_sum(p), pmax = wave_max(pm);
+    const bool fine = fabs(ps - 1.0) <= XSUMTOL && pmax > 0.0 && isfinite(Tc) && isfinite(nd) && nd > 0.0 &&
+                      isfinite(dt) && dt >= 0.0;
+    const int st = Tc < TMIN ? REFUSED_COLD : (Tc > TMAX ? REFUSED_HOT : (fine ? 0 : REFUSED_BAD_INPUT));
+    if (lane == 0) status[cell] = st;
+    if (st == 0) cell_rates(ws, Tc, nd, net, Jc, lane);
+    const double hin = hio[cell];
+    const double h0 = isfinite(hin) ? hin : dt;  // a non-finite h acts as dt, and is counted
+    if (lane == 0 && !isfinite(hin)) atomicAdd(count + COUNT_NONFINITE_H, 1);
+    Step s = {fmax(fmin(h0, dt), H_MIN), 0.0, 0.0, 0.0, 0.0, false};
+    int it = (dt > 0.0 && st == 0) ? 0 : max_steps;  // a refused cell takes no try
+    int ntry = 0;
+    while (it < max_steps && s.t < dt * CALL_END) {
+      const double ha = fmin(s.h, dt - s.t), igh = 1.0 / (GAMMA * ha), ih = 1.0 / ha;
+      // F, w and -Jn at N; a try after a rejection is at the same N and computes the same values again
+      double Fl = 0.0, Ft = 0.0, ul = 0.0, ut = 0.0, u0l = 0.0, u0t = 0.0;
+      const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+      jacobian(c, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
+      add_diagonal(c, igh, ws, lane);
+      double dl, dtr;
+      factor(c, dl, dtr, ws, lane);
+      double y2l[2] = {Fl, ul}, y2t[2] = {Ft, ut};
+      solve<2>(y2l, y2t, c, dl, dtr, ws, lane);
+      const double Ul = y2l[1], Ut = y2t[1];  // M0^-1 w
+      const double c0 = 1.0 / (1.0 - vsum(Ul, Ut, lane));
+      double K1l = y2l[0], K1t = y2t[0];
+      double sk = vsum(K1l, K1t, lane) * c0;
+      K1l += sk * Ul;
+      K1t += sk * Ut;
+      double y1l[1] = {fma(RODAS_C21 * ih, K1l, Fl)}, y1t[1] = {fma(RODAS_C21 * ih, K1t, Ft)};
+      solve<1>(y1l, y1t, c, dl, dtr, ws, lane);
+      sk = vsum(y1l[0], y1t[0], lane) * c0;
+      const double K2l = y1l[0] + sk * Ul, K2t = y1t[0] + sk * Ut;
+      double Yl = Nl + 2.0 * K1l, Yt = Nt + 2.0 * K1t;
+      rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+      y1l[0] = fma(ih, K1l - K2l, Fl);
+      y1t[0] = fma(ih, K1t - K2t, Ft);
+      solve<1>(y1l, y1t, c, dl, dtr, ws, lane);
+      sk = vsum(y1l[0], y1t[0], lane) * c0;
+      const double K3l = y1l[0] + sk * Ul, K3t = y1t[0] + sk * Ut;
+      Yl = Nl + 2.0 * K1l + K3l;
+      Yt = Nt + 2.0 * K1t + K3t;
+      rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+      y1l[0] = fma(ih, fma(-RODAS_C43, K3l, K1l - K2l), Fl);
+      y1t[0] = fma(ih, fma(-RODAS_C43, K3t, K1t - K2t), Ft);
+      solve<1>(y1l, y1t, c, dl, dtr, ws, lane);
+      sk = vsum(y1l[0], y1t[0], lane) * c0;
+      const double K4l = y1l[0] + sk * Ul, K4t = y1t[0] + sk * Ut;
+      bool ok = false;
+      const double en = try_error(Nl, Nt, K1l, K1t, K3l, K3t, K4l, K4t, Yl, Yt, s.t == 0.0, rtol, atol, lane, ok);
+      if (ok) {  // accepted: the new state clamped at 0 (common.cuh:111-118)
+        Nl = fmax(Yl, 0.0);
+        Nt = fmax(Yt, 0.0);
+        s.t += ha;
+        s.nok += 1.0;
+      }
+      const double fac = step_factor(en, ok, s.prej, s.pen, s.pha, ha);
+      s.h = ha * fac;
+      s.prej = !ok;
+      s.pen = en;
+      s.pha = ha;
+      it++;
+      ntry++;
+    }
+    write_state(Nl, Nt, xout + (size_t)cell * NS, net, lane);
+    if (lane == 0) {
+      hio[cell] = s.h;
+      doneo[cell] = (st == 0 && s.t >= dt * CALL_END) ? 1 : 0;  // a refused cell is not done, also at dt = 0
+      noko[cell] = s.nok;
+      if (tries) tries[cell] = (double)ntry;
+    }
+    int nx = 0;
+    if (lane == 0) nx = waves + atomicAdd(count + COUNT_QUEUE, 1);
+    q = __builtin_amdgcn_readfirstlane(nx);
+  }
+}

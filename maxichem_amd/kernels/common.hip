@@ -1,0 +1,508 @@
+// Shared device arithmetic of the chemistry step: one 64-lane wavefront steps one cell.
+//
+// A cell's state vectors are held two values per lane: the lead part (elimination row `lane`, valid for lane < K0)
+// and the trailing part (row K0 + lane, valid for lane < NT). The trailing block of M0 = I / (gamma h) - Jn sits in
+// registers, lane t owning row K0 + t in c[NT]; the lead rows and the trailing rows' lead columns sit dense in the
+// cell's LDS region (LEAD [K0][NS], A21 [NT][K0]). The network header (generated per network) defines the sizes,
+// the LDS layout (L_*), the stream rounds and the method's settings.
+
+typedef unsigned int u32;
+typedef unsigned long long u64;
+
+// Rodas3 (Sandu et al. 1997): gamma = 1/2 and the stage couplings C21 = 4, C43 = -8/3.
+constexpr double GAMMA = 0.5;
+constexpr double RODAS_C21 = 4.0;
+constexpr double RODAS_C43 = 8.0 / 3.0;
+// The Troe factor's published coefficients (Venot et al. 2012, eq. C.2-C.5).
+constexpr double TROE_C0 = 0.4;
+constexpr double TROE_C1 = 0.67;
+constexpr double TROE_N0 = 0.75;
+constexpr double TROE_N1 = 1.27;
+constexpr double TROE_D = 0.14;
+constexpr int REFUSED_COLD = 1;
+constexpr int REFUSED_BAD_INPUT = 2;
+constexpr int REFUSED_HOT = 3;
+
+// Orders the LDS accesses of the wavefront's lanes: the wave executes in lockstep and LDS serves one wave's accesses
+// in issue order, so only the compiler must not move loads and stores across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Lane l's value of v, for a wave-uniform l (v_readlane, no LDS).
+__device__ __forceinline__ double readlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// v from another lane by a DPP pattern (both 32-bit halves), all lanes active.
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Reductions over the 64 lanes in a fixed order, every lane ending with the same bits: within each row of 16 lanes
+// by DPP (pairs, quads, half-rows, rows: each step adds two values in either order, which rounds alike), then the
+// four row results in one wave-uniform sum.
+constexpr int DPP_XOR1 = 0xb1, DPP_XOR2 = 0x4e, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
+
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp<DPP_XOR1>(v);
+  v += dpp<DPP_XOR2>(v);
+  v += dpp<DPP_HALF_MIRROR>(v);
+  v += dpp<DPP_MIRROR>(v);
+  return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  v = fmax(v, dpp<DPP_XOR1>(v));
+  v = fmax(v, dpp<DPP_XOR2>(v));
+  v = fmax(v, dpp<DPP_HALF_MIRROR>(v));
+  v = fmax(v, dpp<DPP_MIRROR>(v));
+  return fmax(fmax(readlane(v, 0), readlane(v, 16)), fmax(readlane(v, 32), readlane(v, 48)));
+}
+
+// The lane index as an opaque value, taken inside each step of an unrolled loop so that the step's lane masks are
+// computed there and not all ahead of the loop, where they would fill the scalar registers.
+__device__ __forceinline__ int opaque_lane(int lane) {
+  asm volatile("" : "+v"(lane));
+  return lane;
+}
+
+// The table pointer p, made opaque to the compiler where it is used, so that the loads and decodes from the
+// per-network tables are not hoisted out of the try loop and held in registers across it.
+template <typename P>
+__device__ __forceinline__ P opaque(P p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+__device__ __forceinline__ bool wave_all(bool p) { return __ballot(!p) == 0ull; }
+
+// A stream word's coefficient: +-1 or +-2 from its sign and x2 bits.
+__device__ __forceinline__ double coefficient(u32 w, int neg, int dbl) {
+  const double c = ((w >> dbl) & 1u) ? 2.0 : 1.0;
+  return ((w >> neg) & 1u) ? -c : c;
+}
+
+// The network's tables in device memory, one pointer for every kernel (a function reads the fields it needs through
+// opaque(net), so they are loaded where used rather than held in registers across the call).
+struct Net {
+  const int* perm;      // [NS] network species of elimination row k
+  const double* rc;     // [NR][RCW] rate-law rows
+  const int* rf;        // [NR] law flags
+  const double* nasa;   // [2][NS][9] NASA-9 ranges, elimination order
+  const u64* rxe;       // [NR] six species (elimination order, pad NS) and the reaction's first dv slot
+  const u32* svw;       // S v stream [SV_ROUNDS][WAVE]
+  const u32* rowp;      // [NS] each row's first S v piece | count << 16
+  const u32* diagw;     // -Jn streams: diagonal, lead, trailing chunks
+  const u32* leadw;
+  const u32* chw;
+  const double* atoms;  // [NS][NEL] network order
+};
+
+// ---- rate coefficients (the reference's law, collision_limit, gibbs_rt and rate_pair: common.cuh:135-242) ----
+
+__device__ __forceinline__ double law(const double* __restrict__ q, int fl, double T, double lT, double iT, double nd) {
+  double k = exp(q[0] + q[1] * lT - q[2] * iT);
+  if (fl & FL_SECOND_K0) k += exp(q[3] + q[4] * lT - q[5] * iT);
+  if (fl & FL_FALLOFF) {
+    const double pr = k * nd / exp(q[6] + q[7] * lT - q[8] * iT);
+    k = k > 0.0 ? k / (1.0 + pr) : 0.0;  // k0 = 0: no rate, also at Pr = 0 / 0; Pr = inf: the high-pressure 0
+#if HAS_TROE
+    if ((fl & FL_TROE) && k > 0.0) {
+      const double fc = log10((1.0 - q[9]) * exp(-T / q[10]) + q[9] * exp(-T / q[11]) + exp(-q[12] / T));
+      const double x = (fl & FL_VM11) ? log10(pr) : log10(pr) - TROE_C0 - TROE_C1 * fc;
+      // Pr = 0: Troe's ratio tends to -1 / d, the VM11 one to -inf (F = 1)
+      const double y = (fl & FL_VM11) ? x / (TROE_N0 - TROE_N1 * fc)
+                                      : (isfinite(x) ? x / (TROE_N0 - TROE_N1 * fc - TROE_D * x) : -1.0 / TROE_D);
+      k *= exp10(fc / (1.0 + y * y));
+    }
+#endif
+  }
+#if HAS_EFF
+  if (fl & FL_EFF) k *= q[13];
+#endif
+  return k;
+}
+
+// A direction's bimolecular-equivalent rate: M-folded k times nd^(molecules - 2); one molecule without M: none.
+__device__ __forceinline__ double bimolecular(int m, bool three_body, double k, double nd) {
+  return m >= 2 ? k * (m == 3 ? nd : 1.0) : (three_body ? k / nd : 0.0);
+}
+
+// G / RT of one species from its NASA-9 range a at T.
+__device__ __forceinline__ double gibbs_rt(const double* __restrict__ a, double T, double lT, double iT) {
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  const double h = -a[0] * iT * iT + a[1] * lT * iT + a[2] + a[3] * T / 2 + a[4] * T2 / 3 + a[5] * T3 / 4 +
+                   a[6] * T4 / 5 + a[7] * iT;
+  const double s = -a[0] * iT * iT / 2 - a[1] * iT + a[2] * lT + a[3] * T + a[4] * T2 / 2 + a[5] * T3 / 3 +
+                   a[6] * T4 / 4 + a[8];
+  return h - s;
+}
+
+__device__ __forceinline__ int species_of(u64 w, int m) { return (int)((w >> (8 * m)) & 0xffull); }
+
+// The rate coefficients of every reaction at the cell's T and nd into KF and KR (LDS), kept there for the call.
+// G / RT (elimination order) goes to the scratch at L_UNION first.
+__device__ __forceinline__ void cell_rates(double* ws, double T, double nd, const Net* net, const double* Jc,
+                                           int lane) {
+  net = opaque(net);
+  const double* rc = net->rc;
+  const int* rf = net->rf;
+  const u64* rxe = net->rxe;
+  double* g = ws + L_UNION;
+  double* kf = ws + L_KF;
+  double* kr = ws + L_KR;
+  const double lT = log(T), iT = 1.0 / T, lKT = log(KBOLTZ * T / P0BAR);
+  const double* cf = net->nasa + (T < NASA_SPLIT_T ? 0 : NS * 9);
+  for (int i = lane; i < NS; i += WAVE) g[i] = gibbs_rt(cf + 9 * i, T, lT, iT);
+  if (lane == 0) g[NS] = 0.0;
+  wave_sync();
+#pragma unroll 1
+  for (int r = lane; r < NR; r += WAVE) {
+    const int fl = rf[r];
+    double f = 0.0, b = 0.0;
+    if (fl & FL_PHOTO) {
+      f = Jc[r - NRT];
+    } else {
+      const u64 w = rxe[r];
+      const double k = law(rc + RCW * r, fl, T, lT, iT, nd);
+      const double dg = g[species_of(w, 3)] + g[species_of(w, 4)] + g[species_of(w, 5)] - g[species_of(w, 0)] -
+                        g[species_of(w, 1)] - g[species_of(w, 2)];
+      const int mf = (fl >> FL_NF_SHIFT) & 3, mr = (fl >> FL_NR_SHIFT) & 3;
+      const bool tb = (fl & FL_THREE_BODY) != 0;
+      const double c3 = tb ? nd : 1.0;
+      f = k * c3;
+      b = (k * exp(dg + (double)(mr - mf) * lKT)) * c3;
+      const double qb = fmax(bimolecular(mf, tb, f, nd), bimolecular(mr, tb, b, nd));
+      if (qb > KCOLL) {  // the collision limit: both directions scaled, their ratio kept
+        const double c = KCOLL / qb;
+        f *= c;
+        b *= c;
+      }
+    }
+    kf[r] = f;
+    kr[r] = b;
+  }
+  wave_sync();
+}
+
+// ---- right-hand side and Jacobian ----
+
+// F = (n_tot / nd) S v(n) at the state Y (lead / trailing parts), u0 = S v / nd; with JAC also the partial
+// derivatives dv/dn of every live reaction into DV. Returns n_tot = sum Y.
+template <bool JAC>
+__device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& Ft, double& u0l, double& u0t,
+                                      const Net* net, double nd, double* ws, int lane) {
+  const double ntot = wave_sum((lane < K0 ? Yl : 0.0) + (lane < NT ? Yt : 0.0));
+  const double xf = nd / ntot, ixf = ntot / nd;
+  double* n = ws + L_NVEC;
+  if (lane < K0) n[lane] = Yl * xf;
+  if (lane < NT) n[K0 + lane] = Yt * xf;
+  if (lane == 0) n[NS] = 1.0;  // the padding species of a shorter side
+  wave_sync();
+  net = opaque(net);
+  const u64* rxe = net->rxe;
+  const int* rf = net->rf;
+  const u32* svw = net->svw;
+  const u32* rowp = net->rowp;
+  const double* kf = ws + L_KF;
+  const double* kr = ws + L_KR;
+  double* V = ws + L_V;
+  double* DV = ws + L_DV;
+#pragma unroll 1
+  for (int r = lane; r < NR; r += WAVE) {
+    const u64 w = rxe[r];
+    const double n0 = n[species_of(w, 0)], n1 = n[species_of(w, 1)], n2 = n[species_of(w, 2)];
+    const double n3 = n[species_of(w, 3)], n4 = n[species_of(w, 4)], n5 = n[species_of(w, 5)];
+    const double f = kf[r], b = kr[r];
+    V[r] = f * n0 * n1 * n2 - b * n3 * n4 * n5;
+    if (JAC) {
+      const int fl = rf[r];
+      if (!(fl & FL_INERT)) {
+        int o = (int)(w >> 48);
+        const int mf = (fl >> FL_NF_SHIFT) & 3, mr = (fl >> FL_NR_SHIFT) & 3;
+        if (mf >= 1) DV[o] = f * n1 * n2;
+        if (mf >= 2) DV[o + 1] = f * n0 * n2;
+        if (mf >= 3) DV[o + 2] = f * n0 * n1;
+        if (!(fl & FL_PHOTO)) {
+          o += mf;
+          if (mr >= 1) DV[o] = b * n4 * n5;
+          if (mr >= 2) DV[o + 1] = b * n3 * n5;
+          if (mr >= 3) DV[o + 2] = b * n3 * n4;
+        }
+      }
+    }
+  }
+  if (lane == 0) V[NR] = 0.0;  // the S v stream's padding term
+  wave_sync();
+  double* part = ws + L_PART;
+  double acc = 0.0;
+#pragma unroll 2
+  for (int q = 0; q < SV_ROUNDS; q++) {
+    const u32 w = svw[q * WAVE + lane];
+    acc = fma(coefficient(w, SV_NEG, SV_DBL), V[w & SV_R_MASK], acc);
+    if ((w >> SV_END) & 1u) {
+      part[w >> SV_PIECE] = acc;
+      acc = 0.0;
+    }
+  }
+  wave_sync();
+  if (lane < K0) {
+    const u32 rp = rowp[lane];
+    double s = 0.0;
+    for (int e = 0; e < (int)(rp >> 16); e++) s += part[(rp & 0xffffu) + e];
+    Fl = s * ixf;
+    u0l = s / nd;
+  }
+  if (lane < NT) {
+    const u32 rp = rowp[K0 + lane];
+    double s = 0.0;
+    for (int e = 0; e < (int)(rp >> 16); e++) s += part[(rp & 0xffffu) + e];
+    Ft = s * ixf;
+    u0t = s / nd;
+  }
+  wave_sync();
+  return ntot;
+}
+
+// One Jacobian stream: each lane sums whole entries of -Jn from dv terms and writes each at its LDS offset.
+__device__ __forceinline__ void jac_stream(const u32* __restrict__ words, int rounds, const double* DV, double* ws,
+                                           int lane) {
+  double acc = 0.0;
+#pragma unroll 2
+  for (int q = 0; q < rounds; q++) {
+    const u32 w = words[q * WAVE + lane];
+    acc = fma(coefficient(w, T_NEG, T_DBL), DV[w & T_IDX_MASK], acc);
+    if ((w >> T_END) & 1u) {
+      ws[w >> T_DEST] = acc;
+      acc = 0.0;
+    }
+  }
+}
+
+// -Jn into LEAD / A21 (LDS) and the trailing block c (registers), from the dv terms that rhs<true> left in DV; then
+// u = w = S v / nd - Jn N / n_tot, the constant-pressure part of the exact Jacobian (lead / trailing parts).
+__device__ __forceinline__ void jacobian(double (&c)[NT], double Nl, double Nt, double u0l, double u0t, double ntot,
+                                         double& ul, double& ut, const Net* net, double* ws, int lane) {
+  double* DV = ws + L_DV;
+  for (int s = lane; s < L_NVEC; s += WAVE) ws[s] = 0.0;  // LEAD and A21
+  if (lane < NT) ws[L_DIAG + lane] = 0.0;                   // a species in no reaction has no diagonal term
+  if (lane == 0) DV[NDV] = 0.0;                             // the padding words' term
+  wave_sync();
+  jac_stream(opaque(net)->diagw, DIAG_ROUNDS, DV, ws, lane);
+  jac_stream(opaque(net)->leadw, LEAD_ROUNDS, DV, ws, lane);
+  double* stage = ws + L_STAGE;
+#pragma unroll
+  for (int ch = 0; ch < NCH; ch++) {
+    if (lane < NT) {
+#pragma unroll
+      for (int j = 0; j < CHUNK; j++) stage[lane * CHUNK + j] = 0.0;
+    }
+    wave_sync();
+    jac_stream(opaque(net)->chw + CH_OFF[ch] * WAVE, CH_ROUNDS[ch], DV, ws, lane);
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < CHUNK; j++)
+      if (ch * CHUNK + j < NT) c[ch * CHUNK + j] = lane < NT ? stage[lane * CHUNK + j] : 0.0;
+  }
+  const double cd = lane < NT ? ws[L_DIAG + lane] : 0.0;
+#pragma unroll
+  for (int j = 0; j < NT; j++) c[j] += j == lane ? cd : 0.0;  // the staging held 0 on the diagonal
+  // the state N into the exchange vector (n's place, free until the next rhs), then w
+  double* xv = ws + L_NVEC;
+  wave_sync();
+  if (lane < K0) xv[lane] = Nl;
+  if (lane < NT) xv[K0 + lane] = Nt;
+  wave_sync();
+  double ml = 0.0, mt = 0.0;
+  if (lane < K0)
+    for (int j = 0; j < NS; j++) ml = fma(ws[lane * NS + j], xv[j], ml);
+  if (lane < NT) {
+    for (int k = 0; k < K0; k++) mt = fma(ws[L_A21 + lane * K0 + k], xv[k], mt);
+#pragma unroll
+    for (int j = 0; j < NT; j++) mt = fma(c[j], xv[K0 + j], mt);
+  }
+  ul = u0l + ml / ntot;
+  ut = u0t + mt / ntot;
+  wave_sync();
+}
+
+// ---- the unpivoted LU of M0 in the elimination order, and its solves ----
+
+// Adds 1 / (gamma h) to M0's diagonal: the lead rows in LDS, the trailing block in registers.
+__device__ __forceinline__ void add_diagonal(double (&c)[NT], double igh, double* ws, int lane) {
+  if (lane < K0) ws[lane * NS + lane] += igh;
+#pragma unroll
+  for (int j = 0; j < NT; j++) c[j] += j == lane ? igh : 0.0;
+  wave_sync();
+}
+
+// LU of the lead columns (right-looking, dense in LDS), the Schur update of the trailing block, then the trailing
+// block's LU in registers. dl: 1 / U_ii of lead row lane; dt: of trailing row lane.
+__device__ __forceinline__ void factor(double (&c)[NT], double& dl, double& dt, double* ws, int lane) {
+  double* A21 = ws + L_A21;
+  dl = 1.0;
+#pragma unroll 1
+  for (int k = 0; k < K0; k++) {
+    const double dk = 1.0 / ws[k * NS + k];
+    if (lane == k) dl = dk;
+    if (lane > k && lane < K0) ws[lane * NS + k] *= dk;
+    if (lane < NT) A21[lane * K0 + k] *= dk;
+    wave_sync();
+    for (int i = k + 1; i < K0; i++) {
+      const double l = ws[i * NS + k];
+      for (int j = k + 1 + lane; j < NS; j += WAVE) ws[i * NS + j] = fma(-l, ws[k * NS + j], ws[i * NS + j]);
+    }
+    if (lane < NT) {
+      const double a = A21[lane * K0 + k];
+      for (int j = k + 1; j < K0; j++) A21[lane * K0 + j] = fma(-a, ws[k * NS + j], A21[lane * K0 + j]);
+    }
+    wave_sync();
+  }
+  if (lane < NT) {
+#pragma unroll 1
+    for (int k = 0; k < K0; k++) {
+      const double a = A21[lane * K0 + k];
+      const double* uk = ws + k * NS + K0;
+#pragma unroll
+      for (int j = 0; j < NT; j++) c[j] = fma(-a, uk[j], c[j]);
+    }
+  }
+  // the trailing block: at pivot k, lane k publishes its row's tail in the row buffer (LDS, the diagonal's place,
+  // free after the Jacobian), which every lane then reads as a broadcast
+  double* row = ws + L_ROW;
+  dt = 1.0;
+#pragma unroll
+  for (int k = 0; k < NT; k++) {
+    if (lane == k) {
+#pragma unroll
+      for (int j = k; j < NT; j++) row[j] = c[j];
+    }
+    wave_sync();
+    const double dk = 1.0 / row[k];
+    if (lane == k) dt = dk;
+    // below the pivot c[k] becomes the multiplier; on and above it stays (times 1, exactly): a product, not a
+    // select, so that the compiler keeps one copy of the row
+    const bool below = opaque_lane(lane) > k;
+    c[k] *= below ? dk : 1.0;
+    const double l = below ? c[k] : 0.0;
+#pragma unroll
+    for (int j = k + 1; j < NT; j++) c[j] = fma(-l, row[j], c[j]);
+    wave_sync();
+  }
+}
+
+// y <- M0^-1 y for NV vectors (lead / trailing parts), with factor's factors.
+template <int NV>
+__device__ __forceinline__ void solve(double (&yl)[NV], double (&yt)[NV], const double (&c)[NT], double dl, double dt,
+                                      const double* ws, int lane) {
+  const double* A21 = ws + L_A21;
+#pragma unroll 1
+  for (int k = 0; k < K0; k++) {  // unit L, lead columns
+    const double lk = (lane > k && lane < K0) ? ws[lane * NS + k] : 0.0;
+    const double tk = lane < NT ? A21[lane * K0 + k] : 0.0;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const double yk = readlane(yl[v], k);
+      yl[v] = fma(-lk, yk, yl[v]);
+      yt[v] = fma(-tk, yk, yt[v]);
+    }
+  }
+  // the trailing block's solves mask the update, not the factor entry: a masked copy of c[k] would depend on the
+  // factors alone, and the compiler would compute all of them ahead of the chain and hold them in registers
+#pragma unroll
+  for (int k = 0; k < NT; k++) {  // unit L, trailing block
+    const bool below = opaque_lane(lane) > k;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const double yk = readlane(yt[v], k);
+      yt[v] = below ? fma(-c[k], yk, yt[v]) : yt[v];
+    }
+  }
+#pragma unroll
+  for (int k = NT - 1; k >= 0; k--) {  // U, trailing block: lane k's value is final here, and it scales it
+    const bool above = opaque_lane(lane) < k;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const double zk = readlane(yt[v] * dt, k);
+      yt[v] = above ? fma(-c[k], zk, yt[v]) : yt[v];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NV; v++) yt[v] *= dt;
+  if (K0 > 0) {
+#pragma unroll
+    for (int t = 0; t < NT; t++) {  // U of the lead rows, trailing columns
+      const double a = lane < K0 ? ws[lane * NS + K0 + t] : 0.0;
+#pragma unroll
+      for (int v = 0; v < NV; v++) yl[v] = fma(-a, readlane(yt[v], t), yl[v]);
+    }
+#pragma unroll 1
+    for (int k = K0 - 1; k >= 0; k--) {  // U, lead columns
+      const double uk = lane < k ? ws[lane * NS + k] : 0.0;
+#pragma unroll
+      for (int v = 0; v < NV; v++) yl[v] = fma(-uk, readlane(yl[v] * dl, k), yl[v]);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) yl[v] *= dl;
+  }
+}
+
+__device__ __forceinline__ double vsum(double yl, double yt, int lane) {
+  return wave_sum((lane < K0 ? yl : 0.0) + (lane < NT ? yt : 0.0));
+}
+
+// ---- step control (the reference's step_factor, next_try and try_error: common.cuh:87-133, 568-589) ----
+
+struct Step {
+  double h, t, nok, pen, pha;
+  bool prej;
+};
+
+__device__ __forceinline__ double step_factor(double en, bool ok, bool prej, double pen, double pha, double ha) {
+  const bool efin = isfinite(en);
+  double fac = efin ? STEP_SAFETY / cbrt(fmax(en, ERR_FLOOR)) : FAC_NONFINITE;
+  fac = fmin(fmax(fac, ok ? FACACC : FACREJ), FACMAX);
+  if (!ok && efin && prej && en > 1.0 && pen > 1.0 && pha > ha) {  // secant: err ~ h^pe through the two tries
+    const double pe = log(pen / en) / log(pha / ha);
+    fac = pe > SECANT_MIN_ORDER ? fmax(pow(SECANT_TARGET / en, 1.0 / fmin(pe, SECANT_MAX_ORDER)), SECMIN) : SECMIN;
+  }
+  if (!ok) fac = fmin(fac, FAC_AFTER_REJECT);
+  return fac;
+}
+
+// The try's error (RMS; while no step of the call is accepted, the larger of RMS and max-norm) and its new state y.
+__device__ __forceinline__ double try_error(double Nl, double Nt, double K1l, double K1t, double K3l, double K3t,
+                                            double K4l, double K4t, double& yl, double& yt, bool first, double rtol,
+                                            double atol, int lane, bool& ok) {
+  yl = Nl + 2.0 * K1l + K3l + K4l;
+  yt = Nt + 2.0 * K1t + K3t + K4t;
+  const double ql = K4l / (atol + rtol * fmax(fabs(Nl), fabs(yl)));
+  const double qt = K4t / (atol + rtol * fmax(fabs(Nt), fabs(yt)));
+  const bool hl = lane < K0, ht = lane < NT;
+  const double e2 = wave_sum((hl ? ql * ql : 0.0) + (ht ? qt * qt : 0.0));
+  const double em = wave_max(fmax(hl ? fabs(ql) : 0.0, ht ? fabs(qt) : 0.0));
+  const bool nn = wave_all((!hl || yl >= -atol) && (!ht || yt >= -atol));
+  const double er = sqrt(e2 / NS);
+  const double en = (first && isfinite(er)) ? fmax(er, em) : er;
+  ok = (en <= 1.0) && nn && isfinite(en);
+  return en;
+}
+
+// x = N / sum N at the cell's output row (network order); returns the sum.
+__device__ __forceinline__ void write_state(double Nl, double Nt, double* __restrict__ out, const Net* net,
+                                            int lane) {
+  const int* perm = opaque(net)->perm;
+  const double s = vsum(Nl, Nt, lane);
+  if (lane < K0) out[perm[lane]] = Nl / s;
+  if (lane < NT) out[perm[K0 + lane]] = Nt / s;
+}

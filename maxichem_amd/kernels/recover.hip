@@ -1,0 +1,211 @@
+// The recovery kernel: recomputes the cells the guard listed, one wavefront per cell, from their inputs, with the
+// call kernel's rates, rhs, Jacobian, stages and step control, but the dense W = I / (gamma h) - Jn - w 1^T factored
+// by LU with threshold partial pivoting (the reference's recover kernel, maxichem/kernels/recover.cuh:3-310).
+
+// Dense LU of W (NS x NS, row-major, LDS) in place: row k is exchanged with the row of the largest |entry| of column k
+// (ties to the lower row; a NaN is never chosen) only when |W_kk| < PIVTAU times it or is not finite. pr[i]: the
+// original row now in row i; dinv[i] = 1 / U_ii.
+__device__ void dense_lu(double* W, int* pr, double* dinv, int lane) {
+  for (int i = lane; i < NS; i += WAVE) pr[i] = i;
+  wave_sync();
+  for (int k = 0; k < NS; k++) {
+    double best = -1.0;
+    int bi = NS;
+    for (int i = k + lane; i < NS; i += WAVE) {
+      const double a = fabs(W[i * NS + k]);
+      if (a > best) {
+        best = a;
+        bi = i;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    const bool exchange = bi != NS && bi != k && !(fabs(W[k * NS + k]) >= PIVTAU * best);
+    wave_sync();
+    if (exchange) {
+      for (int j = lane; j < NS; j += WAVE) {
+        const double a = W[k * NS + j];
+        W[k * NS + j] = W[bi * NS + j];
+        W[bi * NS + j] = a;
+      }
+      if (lane == 0) {
+        const int t = pr[k];
+        pr[k] = pr[bi];
+        pr[bi] = t;
+      }
+    }
+    wave_sync();
+    const double dk = 1.0 / W[k * NS + k];
+    if (lane == 0) dinv[k] = dk;
+    for (int i = k + 1 + lane; i < NS; i += WAVE) {
+      const double l = W[i * NS + k] * dk;
+      W[i * NS + k] = l;
+      for (int j = k + 1; j < NS; j++) W[i * NS + j] = fma(-l, W[k * NS + j], W[i * NS + j]);
+    }
+    wave_sync();
+  }
+}
+
+// y <- W^-1 y with dense_lu's factors; y is the vector in LDS (elimination order).
+__device__ void dense_solve(const double* W, const int* pr, const double* dinv, double* y, int lane) {
+  double a[2];
+#pragma unroll
+  for (int p = 0; p < 2; p++) a[p] = lane + WAVE * p < NS ? y[pr[lane + WAVE * p]] : 0.0;
+  for (int k = 0; k < NS - 1; k++) {
+    const double yk = readlane(k < WAVE ? a[0] : a[1], k & (WAVE - 1));
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+      const int i = lane + WAVE * p;
+      if (i > k && i < NS) a[p] = fma(-W[i * NS + k], yk, a[p]);
+    }
+  }
+  for (int k = NS - 1; k > 0; k--) {
+    const double zk = readlane(k < WAVE ? a[0] : a[1], k & (WAVE - 1)) * dinv[k];
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+      const int i = lane + WAVE * p;
+      if (i < k) a[p] = fma(-W[i * NS + k], zk, a[p]);
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int p = 0; p < 2; p++)
+    if (lane + WAVE * p < NS) y[lane + WAVE * p] = a[p] * dinv[lane + WAVE * p];
+  wave_sync();
+}
+
+// A lead / trailing vector pair into the LDS vector y (elimination order) and back.
+__device__ __forceinline__ void to_lds(double* y, double vl, double vt, int lane) {
+  if (lane < K0) y[lane] = vl;
+  if (lane < NT) y[K0 + lane] = vt;
+  wave_sync();
+}
+
+__device__ __forceinline__ void from_lds(const double* y, double& vl, double& vt, int lane) {
+  vl = lane < K0 ? y[lane] : 0.0;
+  vt = lane < NT ? y[K0 + lane] : 0.0;
+  wave_sync();
+}
+
+extern "C" __global__ void __launch_bounds__(WAVE)
+    mxc_recover(const Net* __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
+                const double* __restrict__ Pa, const double* __restrict__ dta, const double* __restrict__ hin,
+                double* __restrict__ xout, double* __restrict__ hio, int* __restrict__ doneo,
+                double* __restrict__ noko, int* __restrict__ redo, const int* __restrict__ list,
+                int* __restrict__ count, double* __restrict__ tries, const double* __restrict__ Jin, double rtol,
+                double atol, int max_steps) {
+  const int nfl = count[COUNT_RECOMPUTED];
+  if ((int)blockIdx.x >= nfl) return;
+  extern __shared__ double lds[];
+  const int lane = (int)threadIdx.x;
+  double* ws = lds;
+  double* W = lds + L_SIZE;
+  double* dinv = W + NS * NS;
+  double* yv = dinv + NS;
+  double* xo = yv + NS;
+  int* pr = (int*)(xo + NS);
+  double c[NT];
+  int q = (int)blockIdx.x;
+  while (q < nfl) {
+    const int cell = list[q];
+    const double Tc = Ta[cell], dt = dta[cell], nd = Pa[cell] / (KBOLTZ * Tc);
+    const double* xr = x0 + (size_t)cell * NS;
+    const double* Jc = NJ ? Jin + (size_t)cell * NJ : nullptr;
+    // a listed cell passed the call kernel's screen, so it is not screened again
+    cell_rates(ws, Tc, nd, net, Jc, lane);
+    const double h0 = isfinite(hin[cell]) ? hin[cell] : dt;
+    const double m = fmin(h0, dt);
+    int ntry = 0;
+    for (int pass = 0;; pass++) {
+      const int* perm = opaque(net)->perm;
+      double Nl = lane < K0 ? xr[perm[lane]] : 0.0, Nt = lane < NT ? xr[perm[K0 + lane]] : 0.0;
+      const double start = pass == 0 ? m : fmin(m, dt * pow(REC_RETRY_SPLIT, (double)pass));
+      Step s = {fmax(start, H_MIN), 0.0, 0.0, 0.0, 0.0, false};
+      int it = dt > 0.0 ? 0 : max_steps;
+      while (it < max_steps && s.t < dt * CALL_END) {
+        const double ha = fmin(s.h, dt - s.t), igh = 1.0 / (GAMMA * ha), ih = 1.0 / ha;
+        double Fl = 0.0, Ft = 0.0, ul = 0.0, ut = 0.0, u0l = 0.0, u0t = 0.0;
+        const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+        jacobian(c, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
+        // W = I / (gamma h) - Jn - w 1^T, dense (recover.cuh:124-130)
+        to_lds(yv, ul, ut, lane);
+        for (int e = lane; e < K0 * NS; e += WAVE) {
+          const int i = e / NS;
+          W[e] = (e == i * (NS + 1) ? ws[e] + igh : ws[e]) - yv[i];
+        }
+        if (lane < NT) {
+          double* row = W + (K0 + lane) * NS;
+          for (int k = 0; k < K0; k++) row[k] = ws[L_A21 + lane * K0 + k] - ut;
+#pragma unroll
+          for (int j = 0; j < NT; j++) row[K0 + j] = (j == lane ? c[j] + igh : c[j]) - ut;
+        }
+        wave_sync();
+        dense_lu(W, pr, dinv, lane);
+        double K1l, K1t, K2l, K2t, K3l, K3t, K4l, K4t;
+        to_lds(yv, Fl, Ft, lane);
+        dense_solve(W, pr, dinv, yv, lane);
+        from_lds(yv, K1l, K1t, lane);
+        to_lds(yv, fma(RODAS_C21 * ih, K1l, Fl), fma(RODAS_C21 * ih, K1t, Ft), lane);
+        dense_solve(W, pr, dinv, yv, lane);
+        from_lds(yv, K2l, K2t, lane);
+        double Yl = Nl + 2.0 * K1l, Yt = Nt + 2.0 * K1t;
+        rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+        to_lds(yv, fma(ih, K1l - K2l, Fl), fma(ih, K1t - K2t, Ft), lane);
+        dense_solve(W, pr, dinv, yv, lane);
+        from_lds(yv, K3l, K3t, lane);
+        Yl = Nl + 2.0 * K1l + K3l;
+        Yt = Nt + 2.0 * K1t + K3t;
+        rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+        to_lds(yv, fma(ih, fma(-RODAS_C43, K3l, K1l - K2l), Fl), fma(ih, fma(-RODAS_C43, K3t, K1t - K2t), Ft), lane);
+        dense_solve(W, pr, dinv, yv, lane);
+        from_lds(yv, K4l, K4t, lane);
+        bool ok = false;
+        const double en = try_error(Nl, Nt, K1l, K1t, K3l, K3t, K4l, K4t, Yl, Yt, s.t == 0.0, rtol, atol, lane, ok);
+        if (ok) {
+          Nl = fmax(Yl, 0.0);
+          Nt = fmax(Yt, 0.0);
+          s.t += ha;
+          s.nok += 1.0;
+        }
+        const double fac = step_factor(en, ok, s.prej, s.pen, s.pha, ha);
+        s.h = ha * fac;
+        s.prej = !ok;
+        s.pen = en;
+        s.pha = ha;
+        it++;
+        ntry++;
+      }
+      // the result into LDS (network order), where the guard's test reads it
+      const double sum = vsum(Nl, Nt, lane);
+      if (lane < K0) xo[perm[lane]] = Nl / sum;
+      if (lane < NT) xo[perm[K0 + lane]] = Nt / sum;
+      wave_sync();
+      const int dn = s.t >= dt * CALL_END ? 1 : 0;
+      int code = 0;
+      if (lane == 0) code = check_cell(xr, xo, dn, s.nok, opaque(net)->atoms, atol);
+      code = __builtin_amdgcn_readfirstlane(code);
+      wave_sync();
+      if (code == GUARD_DRIFT && pass < REC_RETRIES) continue;  // only drifted: again, with a smaller first try
+      for (int i = lane; i < NS; i += WAVE) xout[(size_t)cell * NS + i] = xo[i];
+      int nx = 0;
+      if (lane == 0) {
+        hio[cell] = s.h;
+        doneo[cell] = code ? 0 : dn;  // done: finished and passing the guard
+        noko[cell] = s.nok;
+        redo[cell] = code ? 2 : 1;
+        if (tries) tries[cell] += (double)ntry;
+        atomicAdd(count + (code ? COUNT_STILL_FAILED : COUNT_RECOVERED), 1);
+        nx = (int)gridDim.x + atomicAdd(count + COUNT_RECOVERY_QUEUE, 1);
+      }
+      q = __builtin_amdgcn_readfirstlane(nx);
+      break;
+    }
+  }
+}

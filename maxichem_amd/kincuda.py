@@ -1,0 +1,193 @@
+"""CudaSolver: the batched Rodas3 chemistry step on one MI355X, a drop-in for the reference's CudaSolver.
+
+The call, its arguments, results, counts and refusals are the reference's (maxichem/kincuda.py:72-361); the kernels
+are this package's HIP kernels (kernels/*.hip), compiled at run time by hiprtc for gfx950 and specialised to the
+network, launched through the C ABI of libmaxichem_amd.so. The name keeps the reference's module and class names so a
+caller's imports change only in the package name.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import NamedTuple
+
+import numpy as np
+import torch
+
+from . import _hip, _kernels, _tables
+from ._constants import (
+    ATOL,
+    COUNTS,
+    DEFAULT_MAX_STEPS,
+    DEFAULT_RTOL,
+    DRIFT,
+    REC_STEPS,
+    T_MAX,
+    T_MIN,
+    WAVE,
+    X_SUM_TOL,
+)
+from ._solver import Solver, check_inputs, check_photolysis
+
+__all__ = [
+    "ATOL", "COUNTS", "DRIFT", "REC_STEPS", "T_MAX", "T_MIN", "X_SUM_TOL", "CallResult", "CudaSolver", "LaunchResult",
+]  # fmt: skip
+
+
+class CallResult(NamedTuple):
+    """Per cell: x [B, S] mole fractions, h [B] next step size (s), done [B] bool, steps [B] accepted steps."""
+
+    x: torch.Tensor
+    h: torch.Tensor
+    done: torch.Tensor
+    steps: torch.Tensor
+
+
+class LaunchResult(NamedTuple):
+    """CallResult's fields and flag [B] int32: 0 passed the guard, 1 recomputed and passes, 2 still fails."""
+
+    x: torch.Tensor
+    h: torch.Tensor
+    done: torch.Tensor
+    steps: torch.Tensor
+    flag: torch.Tensor
+
+
+class CudaSolver(Solver):
+    """Batched chemistry solver on one GPU; calls run one at a time on torch's current stream.
+
+    Parameters
+    ----------
+    net : VulcanNetwork
+    device : torch.device or str
+    drift : float, optional
+        The guard's element-drift allowance (DRIFT if None). Keyword only, as photo.
+    photo : bool
+        The network's photolysis branches enter the call, which then takes J [B, branches].
+    """
+
+    def __init__(self, net, device, *, drift: float | None = None, photo: bool = False) -> None:
+        super().__init__(net, device, drift, photo)
+        with torch.cuda.device(self.dev):
+            self._setup(net, photo)
+
+    def _setup(self, net, photo: bool) -> None:
+        tab = _tables.build(net, photo)
+        self.tab = tab
+        self.ns, self.nr, self.nj = tab.ns, tab.nr, tab.nj
+        self.cpw = 1  # cells (wavefronts) per workgroup
+        dev = self.dev.index
+        info = _hip.device_info(dev)
+        lay = tab.layout()
+        self.lds_call = 8 * lay["SIZE"] * self.cpw
+        self.lds_rec = 8 * (lay["SIZE"] + tab.ns * tab.ns + 3 * tab.ns) + 4 * tab.ns
+        for what, need in (("call", self.lds_call), ("recovery", self.lds_rec)):
+            if need > info["lds_per_block"]:
+                raise ValueError(
+                    f"one cell's {what} kernel needs {need} B of LDS per workgroup; the device has "
+                    f"{info['lds_per_block']}"
+                )
+        code = _hip.compile_source(_kernels.source(tab, self.cpw, self.drift))
+        self._module = _hip.Module(code, dev)
+        self._call = self._module.function("mxc_call")
+        self._guard = self._module.function("mxc_guard")
+        self._recover = self._module.function("mxc_recover")
+        cus = info["compute_units"]
+        self.max_blocks = cus * max(1, _hip.occupancy(self._call, WAVE * self.cpw, self.lds_call))
+        self.rec_blocks = cus * max(1, _hip.occupancy(self._recover, WAVE, self.lds_rec))
+        nasa_e = tab.nasa[:, tab.perm, :]  # the kernels index species in elimination order
+        t = {
+            "perm": tab.perm.astype(np.int32),
+            "rc": tab.rc.astype(np.float64),
+            "rf": tab.rf.astype(np.int32),
+            "nasa": np.ascontiguousarray(nasa_e, np.float64),
+            "rxe": tab.rxe.view(np.int64),
+            "svw": tab.sv_words.ravel().view(np.int32),
+            "rowp": tab.row_pieces.view(np.int32),
+            "diagw": tab.diag_words.ravel().view(np.int32),
+            "leadw": tab.lead_words.ravel().view(np.int32),
+            "chw": np.concatenate([w.ravel() for w in tab.chunk_words]).view(np.int32),
+            "atoms": tab.atoms.astype(np.float64),
+        }
+        self._tables = {k: torch.from_numpy(np.ascontiguousarray(v)).to(self.dev) for k, v in t.items()}
+        # the kernels' Net struct: the tables' device pointers, in its field order
+        fields = ("perm", "rc", "rf", "nasa", "rxe", "svw", "rowp", "diagw", "leadw", "chw", "atoms")
+        self._net = torch.tensor([self._tables[k].data_ptr() for k in fields], dtype=torch.int64, device=self.dev)
+        self.tries: torch.Tensor | None = None
+        self.info = {
+            "ns": tab.ns, "nr": tab.nr, "nj": tab.nj, "k0": tab.k0, "nt": tab.nt, "ndv": tab.ndv,
+            "jac_terms": tab.jac_terms, "sv_terms": tab.sv_terms, "lds_call": self.lds_call, "lds_rec": self.lds_rec,
+            "regs": _hip.attribute(self._call, _hip.ATTR_REGISTERS),
+            "private_bytes": _hip.attribute(self._call, _hip.ATTR_PRIVATE_BYTES),
+            "cells_per_cu": self.max_blocks // cus * self.cpw, "max_blocks": self.max_blocks,
+            "rec_blocks": self.rec_blocks, "compute_units": cus, "drift": self.drift, "tables": tab.digest(),
+        }  # fmt: skip
+
+    def launch(
+        self,
+        x: torch.Tensor,
+        T: torch.Tensor,
+        P: torch.Tensor,
+        dt: torch.Tensor,
+        h: torch.Tensor,
+        *,
+        rtol: float = DEFAULT_RTOL,
+        atol: float = ATOL,
+        max_steps: int = DEFAULT_MAX_STEPS,
+        J: torch.Tensor | None = None,
+    ) -> LaunchResult:
+        """The call, the guard and the recovery; reads the counts back (the one host sync) and refuses bad input
+        and T out of range (maxichem/kincuda.py:230-320)."""
+        self.flag = None
+        self._require_open()
+        B = check_inputs(self.dev, self.ns, x, T, P, dt, h, rtol, atol, max_steps)
+        with torch.cuda.device(self.dev):
+            x, T, P, dt, h = (a.contiguous() for a in (x, T, P, dt, h))
+            jx = check_photolysis(J, B, self.nj, self.dev)
+            xo, ho = torch.empty_like(x), h.clone()
+            done, redo, lst, status = (torch.empty(B, dtype=torch.int32, device=self.dev) for _ in range(4))
+            nok = torch.empty(B, dtype=torch.float64, device=self.dev)
+            self.tries = torch.zeros(B, dtype=torch.float64, device=self.dev)
+            self.count.zero_()
+            stream = torch.cuda.current_stream(self.dev).cuda_stream
+            self._launch_all(x, T, P, dt, h, xo, ho, done, nok, status, redo, lst, jx, B, rtol, atol, max_steps,
+                             stream, None)  # fmt: skip
+            self.total += self.count
+            self._refuse(x, T, P, dt, jx, status)
+        self.flag = redo
+        return LaunchResult(xo, ho, done.bool(), nok, redo)
+
+    def _launch_all(self, x, T, P, dt, h, xo, ho, done, nok, status, redo, lst, jx, B, rtol, atol, max_steps,
+                    stream, order) -> None:  # fmt: skip
+        p = _hip.ptr
+        net = p(self._net)
+        call_args = [net, p(x), p(T), p(P), p(dt), p(ho), p(xo), p(done), p(nok), p(status), p(self.count),
+                     p(self.tries), p(order), p(jx), ctypes.c_int(B), ctypes.c_double(rtol), ctypes.c_double(atol),
+                     ctypes.c_int(max_steps)]  # fmt: skip
+        grid = max(1, min(-(-B // self.cpw), self.max_blocks))
+        _hip.launch(self._call, grid, WAVE * self.cpw, self.lds_call, stream, call_args)
+        guard_args = [net, p(x), p(xo), p(done), p(nok), p(status), p(redo), p(lst), p(self.count), ctypes.c_int(B),
+                      ctypes.c_double(atol)]  # fmt: skip
+        _hip.launch(self._guard, max(1, -(-B // _kernels.GUARD_THREADS)), _kernels.GUARD_THREADS, 0, stream,
+                    guard_args)  # fmt: skip
+        rec_args = [net, p(x), p(T), p(P), p(dt), p(h), p(xo), p(ho), p(done), p(nok), p(redo), p(lst), p(self.count),
+                    p(self.tries), p(jx), ctypes.c_double(rtol), ctypes.c_double(atol),
+                    ctypes.c_int(REC_STEPS * max_steps)]  # fmt: skip
+        _hip.launch(self._recover, max(1, min(B, self.rec_blocks)), WAVE, self.lds_rec, stream, rec_args)
+
+    def __call__(
+        self,
+        x: torch.Tensor,
+        T: torch.Tensor,
+        P: torch.Tensor,
+        dt: torch.Tensor,
+        h: torch.Tensor,
+        *,
+        rtol: float = DEFAULT_RTOL,
+        atol: float = ATOL,
+        max_steps: int = DEFAULT_MAX_STEPS,
+        J: torch.Tensor | None = None,
+    ) -> CallResult:
+        """Advance each cell by dt: launch without the flag in its result."""
+        r = self.launch(x, T, P, dt, h, rtol=rtol, atol=atol, max_steps=max_steps, J=J)
+        return CallResult(r.x, r.h, r.done, r.steps)

@@ -1,0 +1,116 @@
+"""The HIP path against the CPU oracle on the same seeded inputs (GPU).
+
+The bar (the north star's): species whose oracle mole fraction is above 1e-20 agree to relative 1e-2, element shares
+move by at most 1e-10 per call, and done flags match. Each case also prints how close the two are.
+"""
+
+import helpers
+import numpy as np
+import oracle
+import pytest
+import torch
+
+from maxichem_amd import _tables, standin, workloads
+from maxichem_amd.kincuda import CudaSolver
+
+pytestmark = pytest.mark.gpu
+REL = 1e-2
+FLOOR = 1e-20
+SHARE_DRIFT = 1e-10
+
+
+def t(a):
+    return torch.tensor(np.asarray(a), dtype=torch.float64, device="cuda")
+
+
+def shares(x, atoms):
+    e = np.asarray(x) @ atoms
+    return e / e.sum(1, keepdims=True)
+
+
+def compare(name, got, want, x0, atoms):
+    xg, xw = got["x"], want["x"]
+    big = xw > FLOOR
+    rel = np.abs(xg[big] / xw[big] - 1.0)
+    s0, s1 = shares(x0, atoms), shares(xg, atoms)
+    drift = np.abs(s1 - s0) / np.maximum(s0, 1e-300)
+    drift = np.where(s0 > 1e-12, drift, 0.0)
+    steps = np.abs(got["steps"] - want["steps"]) / np.maximum(want["steps"], 1.0)
+    print(f"{name}: max rel {rel.max():.2e}, p99 {np.percentile(rel, 99):.2e}; share drift {drift.max():.1e}; "
+          f"done {int(got['done'].sum())}/{int(want['done'].sum())}; steps rel diff max {steps.max():.2e}")  # fmt: skip
+    assert np.array_equal(got["done"], want["done"])
+    assert rel.max() <= REL
+    assert drift.max() <= SHARE_DRIFT
+
+
+def run_gpu(solver, inp, dt_key="dt", J=True):
+    r = solver.launch(t(inp["x"]), t(inp["T"]), t(inp["P"]), t(inp[dt_key]), t(inp["h"]),
+                      J=t(inp["J"]) if J else None)  # fmt: skip
+    return {k: v.cpu().numpy() for k, v in r._asdict().items()}
+
+
+@pytest.mark.parametrize("shape", ["NCHO", "SNCHO"])
+def test_config0_cells_match_the_oracle(shape):
+    net = standin.load(shape)
+    inp = workloads.config0(net.species, len(net.photo), cells=256, seed=0)
+    tab = _tables.build(net, True)
+    want = oracle.Oracle(tab).call(inp["x"], inp["T"], inp["P"], inp["dt"], inp["h"], J=inp["J"])
+    with CudaSolver(net, "cuda", photo=True) as solver:
+        got = run_gpu(solver, inp)
+        print(solver.info)
+    compare(f"{shape} config0", got, want, inp["x"], net.atoms)
+
+
+def test_config1_relax_then_lit_calls_match_the_oracle():
+    net = standin.load("NCHO")
+    inp = workloads.config1(net.species, len(net.photo), cells=512, seed=1)
+    orc = oracle.Oracle(_tables.build(net, True))
+    with CudaSolver(net, "cuda", photo=True) as solver:
+        x, h = inp["x"], inp["h"]
+        zero = np.zeros_like(inp["J"])
+        want = orc.call(x, inp["T"], inp["P"], inp["dt_relax"], h, J=zero)
+        got = run_gpu(solver, {**inp, "x": x, "h": h, "J": zero}, "dt_relax")
+        compare("NCHO config1 relax (dt 1e8 s)", got, want, x, net.atoms)
+        x, h = want["x"], want["h"]  # continue both from the oracle's state, so each call is compared alone
+        for call in range(3):
+            want = orc.call(x, inp["T"], inp["P"], inp["dt"], h, J=inp["J"])
+            got = run_gpu(solver, {**inp, "x": x, "h": h})
+            compare(f"NCHO config1 call {call + 1} (dt 300 s)", got, want, x, net.atoms)
+            x, h = want["x"], want["h"]
+
+
+@pytest.mark.parametrize("stem", ["synthetic", "synthetic_small", "synthetic_falloff"])
+def test_synthetic_networks_match_the_oracle(read_network, stem):
+    net = read_network(stem)
+    photo = bool(net.photo)
+    rng = np.random.default_rng(3)
+    cells = 64
+    x = 10.0 ** rng.uniform(-10, 0, (cells, len(net.species)))
+    x /= x.sum(1, keepdims=True)
+    T = 10.0 ** rng.uniform(np.log10(300), np.log10(3000), cells)
+    P = 10.0 ** rng.uniform(2, 9, cells)
+    dt = 10.0 ** rng.uniform(-3, 6, cells)
+    h = np.full(cells, 1e-8)
+    J = 10.0 ** rng.uniform(-6, -2, (cells, len(net.photo))) if photo else None
+    want = oracle.Oracle(_tables.build(net, photo)).call(x, T, P, dt, h, J=J)
+    with CudaSolver(net, "cuda", photo=photo) as solver:
+        r = solver.launch(t(x), t(T), t(P), t(dt), t(h), J=None if J is None else t(J))
+        got = {k: v.cpu().numpy() for k, v in r._asdict().items()}
+    compare(stem, got, want, x, net.atoms)
+
+
+def test_results_do_not_depend_on_the_batch():
+    net = standin.load("NCHO")
+    inp = workloads.config0(net.species, len(net.photo), cells=512, seed=7)
+    args = [t(inp[k]) for k in ("x", "T", "P", "dt", "h")]
+    J = t(inp["J"])
+    with CudaSolver(net, "cuda", photo=True) as solver:
+        big = solver.launch(*args, J=J)
+        perm = torch.randperm(512, generator=torch.Generator().manual_seed(3)).cuda()
+        shuffled = solver.launch(*(a[perm] for a in args), J=J[perm])
+        sel = torch.arange(0, 512, 16, device="cuda")
+        alone = solver.launch(*(a[sel] for a in args), J=J[sel])
+    inverse = torch.empty_like(perm)
+    inverse[perm] = torch.arange(512, device="cuda")
+    assert all(torch.equal(p, q[inverse]) for p, q in zip(big, shuffled))
+    assert all(torch.equal(p[sel], q) for p, q in zip(big, alone))

@@ -1,0 +1,74 @@
+"""The kernel tables on the CPU: the LDS layout's live regions stay apart, and the streams decode back to the
+network's S v and -Jn (the reference checks its streams the same way, tests/test_tables_source.py)."""
+
+import numpy as np
+import pytest
+
+from maxichem_amd import _tables, standin
+from maxichem_amd._tables import CHUNK, SV_END, SV_NEG, SV_DBL, SV_PIECE, T_DBL, T_DEST, T_END, T_NEG
+
+
+def _nets(read_network):
+    return [(read_network(s), p) for s in ("synthetic", "synthetic_small", "synthetic_decay") for p in (False, True)
+            ] + [(standin.load("NCHO"), True), (standin.load("SNCHO"), True)]  # fmt: skip
+
+
+def test_layout_regions_that_live_together_do_not_overlap(read_network):
+    for net, photo in _nets(read_network):
+        tab = _tables.build(net, photo)
+        L = tab.layout()
+        regions = {
+            "lead": (0, L["A21"]), "a21": (L["A21"], L["NVEC"]), "n": (L["NVEC"], L["NVEC"] + tab.ns + 1),
+            "dv": (L["DV"], L["DV"] + tab.ndv + 1), "v": (L["V"], L["V"] + tab.nr + 1),
+            "kf": (L["KF"], L["KF"] + tab.nr), "kr": (L["KR"], L["KR"] + tab.nr),
+            "part": (L["PART"], L["PART"] + tab.npart), "diag": (L["DIAG"], L["DIAG"] + tab.nt),
+            "stage": (L["STAGE"], L["STAGE"] + tab.nt * CHUNK),
+        }  # fmt: skip
+        # live at once: the rhs with the Jacobian's dv; the Jacobian's assembly; the factorization
+        groups = [("lead", "a21", "n", "kf", "kr", "dv", "v", "part"),
+                  ("lead", "a21", "n", "kf", "kr", "dv", "diag", "stage")]
+        for g in groups:
+            spans = sorted(regions[k] for k in g)
+            for (a0, a1), (b0, b1) in zip(spans, spans[1:]):
+                assert a1 <= b0, (g, spans)
+        assert max(b for _, b in regions.values()) <= L["SIZE"]
+        assert L["UNION"] + tab.ns + 1 <= L["SIZE"]
+
+
+def test_streams_decode_to_s_v_and_minus_jn(read_network):
+    for net, photo in _nets(read_network):
+        tab = _tables.build(net, photo)
+        S = np.hstack([net.S, net.Sp]) if photo else np.asarray(net.S)
+        perm, ns = tab.perm, tab.ns
+        # S v: each piece is a signed sum of reactions; each row gathers its pieces
+        pieces = {}
+        for lane in range(64):
+            acc = {}
+            for w in tab.sv_words[:, lane]:
+                r = int(w) & ((1 << 11) - 1)
+                c = (2 if (w >> SV_DBL) & 1 else 1) * (-1 if (w >> SV_NEG) & 1 else 1)
+                if r < tab.nr:
+                    acc[r] = acc.get(r, 0) + c
+                if (w >> SV_END) & 1:
+                    pieces[int(w) >> SV_PIECE] = acc
+                    acc = {}
+        for i in range(ns):
+            first, count = int(tab.row_pieces[i]) & 0xFFFF, int(tab.row_pieces[i]) >> 16
+            row = {}
+            for p in range(first, first + count):
+                for r, c in pieces[p].items():
+                    row[r] = row.get(r, 0) + c
+            want = {r: int(S[perm[i], r]) for r in range(tab.nr) if S[perm[i], r] != 0}
+            assert row == want
+        # -Jn: every destination written once, from dv terms; the number of terms matches
+        dests = []
+        terms = 0
+        for words in [tab.diag_words, tab.lead_words, *tab.chunk_words]:
+            for lane in range(64):
+                for w in words[:, lane]:
+                    if (int(w) & 0xFFFF) != tab.ndv:
+                        terms += 1
+                    if (w >> T_END) & 1:
+                        dests.append((id(words), int(w) >> T_DEST))
+        assert len(dests) == len(set(dests))
+        assert terms == tab.jac_terms

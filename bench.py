@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Cells/s per call of the batched Rodas3 chemistry step on one MI355X.
+
+Workload (the GPU FP64 case): 65,536 cells of the 69-species C-H-N-O network (390 reversible thermal reactions,
+48 photolysis branches; the seeded stand-in of that shape unless VULCAN_THERMO holds the real files). Per cell, from
+numpy.random.default_rng(1): T log-uniform 300-3000 K, P log-uniform 1e2-1e9 dyn cm^-2, a solar gas. One dark call of
+dt = 1e8 s relaxes the cells (set-up, untimed); then W untimed and K timed calls of dt = 300 s follow, each starting
+from the previous call's x and h, with per-branch J log-uniform 1e-9 to 1e-3 s^-1 on the lowest-P quarter of cells
+and J = 0 elsewhere. All float64, inputs resident in HBM.
+
+A step is one call, solver(x, T, P, dt, h, J=J): the call kernel, the guard, the recovery and the count read.
+value = cells x K / wall time of the K calls. The roofline is the call kernel's: its algorithmic FP64 flops per launch
+(per-try work counted from the network's tables, times each cell's tries, which the kernel counts) over its average
+launch duration from HIP events on the launch stream. The CPU baseline times the FP64 oracle (OpenMP, one cell per
+thread) on a sample of the first timed call's cells and checks the GPU's result on them.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+FP64_PEAK = 78.6  # TFLOP/s, MI355X FP64 vector
+
+
+def parse() -> argparse.Namespace:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed calls")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed calls of dt = 300 s after the dark relaxation")
+    ap.add_argument("--cells", type=int, default=65536)
+    ap.add_argument("--network", default="NCHO", choices=("NCHO", "SNCHO"))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="leave out the CPU baseline (profiling runs)")
+    return ap.parse_args()
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(tab, start: dict, first, cells: int, seconds: float) -> dict:
+    """The oracle over an evenly spaced sample of the first timed call's cells, and the GPU's error on them."""
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT))
+    import oracle
+
+    threads = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    orc = oracle.Oracle(tab)
+
+    def run(n: int):
+        idx = np.linspace(0, cells - 1, n).round().astype(np.int64)
+        a = {k: v[idx] for k, v in start.items()}
+        t0 = time.perf_counter()
+        out = orc.call(a["x"], a["T"], a["P"], a["dt"], a["h"], J=a["J"], threads=threads)
+        return idx, out, time.perf_counter() - t0
+
+    n = 4 * threads
+    idx, out, t = run(n)
+    if t < seconds / 4:
+        n = int(min(cells, max(n, n * seconds / max(t, 1e-3))))
+        idx, out, t = run(n)
+    xg, dg = first["x"][idx], first["done"][idx]
+    big = out["x"] > 1e-20
+    rel = float(np.max(np.abs(xg[big] / out["x"][big] - 1.0))) if big.any() else 0.0
+    sh = lambda x: (x @ tab.atoms) / (x @ tab.atoms).sum(1, keepdims=True)  # noqa: E731
+    share = float(np.max(np.abs(sh(xg) - sh(out["x"]))))
+    return {
+        "value": n / t, "unit": "cells/s", "cores": threads, "kind": "port",
+        "sample": f"the FP64 oracle (C, OpenMP) on {n} evenly spaced cells of the first timed call, dt = 300 s, "
+                  f"{t:.1f} s",
+        "gpu_vs_oracle": {"max_rel_species_above_1e-20": rel, "max_share_diff": share,
+                          "done_match": bool((dg == out["done"]).all()), "cells": n},
+    }  # fmt: skip
+
+
+def main() -> int:
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU: the cells of a call are independent and this build measures one GPU's "
+              f"share (asked for --gpus {args.gpus}, WORLD_SIZE {world})", file=sys.stderr)  # fmt: skip
+        return 2
+    import numpy as np
+    import torch
+
+    from maxichem_amd import flops, standin, workloads
+    from maxichem_amd.kincuda import CudaSolver
+
+    dev = torch.device("cuda", 0)
+    net = standin.real_or_standin(args.network)
+    nj = len(net.photo)
+    d = workloads.config1(net.species, nj, args.cells)
+    log(f"network {args.network}: {len(net.species)} species, {len(net.reac)} thermal, {nj} photolysis; "
+        f"{args.cells} cells")  # fmt: skip
+
+    def g(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+    solver = CudaSolver(net, dev, photo=True)
+    log(f"solver: {solver.info}")
+    x, T, P, h, J = g(d["x"]), g(d["T"]), g(d["P"]), g(d["h"]), g(d["J"])
+    dt_relax, dt = g(d["dt_relax"]), g(d["dt"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = solver(x, T, P, dt_relax, h, J=torch.zeros_like(J))
+    torch.cuda.synchronize()
+    log(f"dark relaxation dt = 1e8 s: {time.perf_counter() - t0:.2f} s, done {int(r.done.sum())}/{args.cells}, "
+        f"fallback {solver.fallback}")  # fmt: skip
+    x, h = r.x, r.h
+    for w in range(args.warmup):
+        r = solver(x, T, P, dt, h, J=J)
+        x, h = r.x, r.h
+        log(f"warmup call {w + 1}/{args.warmup} done")
+    start = {"x": x, "T": T, "P": P, "dt": dt, "h": h, "J": J}
+    start = {k: v.cpu().numpy() for k, v in start.items()}
+
+    solver.timing = []
+    tries, first = [], None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        r = solver(x, T, P, dt, h, J=J)
+        x, h = r.x, r.h
+        tries.append(solver.tries)
+        if first is None:
+            first = r
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    events, solver.timing = solver.timing, None
+    kernel_ms = [a.elapsed_time(b) for a, b in events]
+    tries = [t.cpu().numpy() for t in tries]
+    done = int(r.done.sum())
+    log(f"{args.steps} timed calls: {wall:.3f} s; call kernel {np.mean(kernel_ms):.3f} ms average; last call done "
+        f"{done}/{args.cells}; mean tries per cell-call {np.mean([t.mean() for t in tries]):.2f}")  # fmt: skip
+
+    per_launch = np.array([flops.call_flops(solver.tab, t) for t in tries])
+    achieved = float(per_launch.sum() / (np.sum(kernel_ms) * 1e-3) / 1e12)
+    w = flops.work(solver.tab)
+    line = {
+        "metric": "cells/s per call (NCHO, 64K cells): learned+FP32-LU vs plain FP64 at same error",
+        "value": args.cells * args.steps / wall,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.network} {args.cells} cells, plain FP64 form: dark relaxation dt = 1e8 s (untimed), "
+                        "then calls of dt = 300 s carrying x and h, J log-uniform 1e-9..1e-3 s^-1 on the lowest-P "
+                        "quarter",
+            "network": f"{'stand-in ' if getattr(net, 'standin', True) else ''}{args.network}: "
+                       f"{len(net.species)} species, {len(net.reac)} thermal, {nj} photolysis",
+            "cells": args.cells, "seed": 1, "form": "plain FP64", "rtol": 1e-3, "atol": 1e-25,
+        },
+        "roofline": {
+            "bound": "mfma", "compute": "FP64 (vector FMA; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+            "achieved": achieved, "peak": FP64_PEAK, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK, "traffic": None,
+            "kernel": "mxc_call", "kernel_ms": float(np.mean(kernel_ms)),
+            "flop_per_try": w.per_try, "flop_rates_per_cell": w.rates,
+            "tries_per_launch": float(np.mean([t.sum() for t in tries])),
+        },  # fmt: skip
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        first_np = {"x": first.x.cpu().numpy(), "done": first.done.cpu().numpy()}
+        log("CPU baseline: the oracle on a sample of the first timed call")
+        line["cpu_baseline"] = cpu_baseline(solver.tab, start, first_np, args.cells, args.cpu_seconds)
+    solver.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -114,6 +114,8 @@ class CudaSolver(Solver):
         fields = ("perm", "rc", "rf", "nasa", "rxe", "svw", "rowp", "diagw", "leadw", "chw", "atoms")
         self._net = torch.tensor([self._tables[k].data_ptr() for k in fields], dtype=torch.int64, device=self.dev)
         self.tries: torch.Tensor | None = None
+        # a list to collect (start, end) events of each call kernel launch on the call's stream; None: not recorded
+        self.timing: list | None = None
         self.info = {
             "ns": tab.ns, "nr": tab.nr, "nj": tab.nj, "k0": tab.k0, "nt": tab.nt, "ndv": tab.ndv,
             "jac_terms": tab.jac_terms, "sv_terms": tab.sv_terms, "lds_call": self.lds_call, "lds_rec": self.lds_rec,
@@ -165,7 +167,13 @@ class CudaSolver(Solver):
                      p(self.tries), p(order), p(jx), ctypes.c_int(B), ctypes.c_double(rtol), ctypes.c_double(atol),
                      ctypes.c_int(max_steps)]  # fmt: skip
         grid = max(1, min(-(-B // self.cpw), self.max_blocks))
+        if self.timing is not None:
+            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            ev[0].record()
         _hip.launch(self._call, grid, WAVE * self.cpw, self.lds_call, stream, call_args)
+        if self.timing is not None:
+            ev[1].record()
+            self.timing.append(ev)
         guard_args = [net, p(x), p(xo), p(done), p(nok), p(status), p(redo), p(lst), p(self.count), ctypes.c_int(B),
                       ctypes.c_double(atol)]  # fmt: skip
         _hip.launch(self._guard, max(1, -(-B // _kernels.GUARD_THREADS)), _kernels.GUARD_THREADS, 0, stream,

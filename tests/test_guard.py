@@ -1,0 +1,324 @@
+"""The guard, the recovery and the refusals on the GPU: counts, flags, messages and what the other cells get.
+
+Each case follows one of the reference's guard tests (maxichem tests/test_guard.py:26-424) in behaviour: what is
+refused, how it is counted and named, and that the cells left alone are bitwise what they are in a batch without the
+refused ones. Where the recovery recomputes a cell, its result is compared with the oracle's recovery of that cell.
+"""
+
+import ctypes
+
+import numpy as np
+import oracle
+import pytest
+import torch
+
+from maxichem_amd import _hip, _kernels, _tables, standin, workloads
+from maxichem_amd._constants import T_MAX, T_MIN, X_SUM_TOL
+from maxichem_amd.kincuda import CallResult, CudaSolver, LaunchResult
+
+pytestmark = pytest.mark.gpu
+
+
+def t(a):
+    return torch.tensor(np.asarray(a), dtype=torch.float64, device="cuda")
+
+
+def full(n, v):
+    return torch.full((n,), float(v), dtype=torch.float64, device="cuda")
+
+
+@pytest.fixture(scope="module")
+def ncho():
+    if not torch.cuda.is_available():
+        pytest.skip("this test needs a GPU")
+    net = standin.load("NCHO")
+    with CudaSolver(net, "cuda") as solver:
+        yield net, solver
+
+
+def starts(net, solver, cells, seed, settle=100.0):
+    """Solar gas states at seeded T and P, settled by a call of `settle` seconds (its x and h)."""
+    inp = workloads.config0(net.species, 0, cells=cells, seed=seed)
+    x, T, P = t(inp["x"]), t(inp["T"]), t(inp["P"])
+    r = solver(x, T, P, full(cells, settle), full(cells, 1e-8))
+    return r.x, T, P, r.h
+
+
+def same(a, b):
+    return all(torch.equal(p, q) for p, q in zip(a, b))
+
+
+def test_nan_input_is_refused(ncho, monkeypatch):
+    net, solver = ncho
+    cells = 64
+    x, T, P, h = starts(net, solver, cells, 1)
+    x[3, 5] = float("nan")
+    dt = full(cells, 1.0)
+    with pytest.raises(ValueError, match=r"1 cells of bad input: .*\(first: cell 3: sum\(x\) - 1 = nan"):
+        solver(x, T, P, dt, h)
+    assert solver.flag is None
+    with monkeypatch.context() as m:  # what the kernels returned
+        m.setattr(solver, "_refuse", lambda *_: None)
+        y, _, d, _ = solver(x, T, P, dt, h)
+    counts = solver.fallback
+    keep = torch.arange(cells, device="cuda") != 3
+    alone = solver(x[keep], T[keep], P[keep], dt[keep], h[keep])
+    print(f"counts {counts}")
+    assert counts["bad_input"] == 1 and counts["recomputed"] == 0
+    assert not bool(d[3]) and not bool(torch.isfinite(y[3]).all())
+    assert torch.equal(y[keep], alone.x) and bool(d[keep].all())
+
+
+def test_forced_failure_is_recomputed_as_the_oracle_does(ncho):
+    net, solver = ncho
+    cells = 64
+    x, T, P, h = starts(net, solver, cells, 1)
+    inp = workloads.config0(net.species, 0, cells=cells, seed=1)
+    dt = full(cells, 1.0)
+    x[7], h[7], dt[7] = t(inp["x"][7]), 1e-10, 500.0  # cell 7: a cold start over 500 s in 10 tries
+    r = solver.launch(x, T, P, dt, h, max_steps=10)
+    counts = solver.fallback
+    print(f"counts {counts}; cell 7 flag {int(r.flag[7])} done {bool(r.done[7])} steps {float(r.steps[7])}")
+    assert counts["unfinished"] == 1 and counts["recomputed"] == 1
+    assert counts["recovered"] + counts["still_failed"] == 1
+    assert int(r.flag[7]) == (1 if bool(r.done[7]) else 2)
+    others = torch.arange(cells, device="cuda") != 7
+    assert bool(r.done[others].all()) and int(r.flag[others].abs().sum()) == 0
+    orc = oracle.Oracle(_tables.build(net, False))
+    c = [7]
+    want = orc.call(x[c].cpu().numpy(), T[c].cpu().numpy(), P[c].cpu().numpy(), dt[c].cpu().numpy(),
+                    h[c].cpu().numpy(), max_steps=10)  # fmt: skip
+    got = r.x[7].cpu().numpy()
+    big = want["x"][0] > 1e-20
+    rel = np.abs(got[big] / want["x"][0][big] - 1.0).max()
+    print(f"recovered cell against the oracle's recovery: max rel {rel:.2e}, steps {want['steps'][0]}, "
+          f"flag {want['flag'][0]}")  # fmt: skip
+    assert int(want["flag"][0]) == int(r.flag[7]) and bool(want["done"][0]) == bool(r.done[7])
+    assert rel <= 1e-2
+
+
+def test_element_drift_is_recomputed(ncho):
+    # allowance 0 flags every rounding change of an element share; atol 1e-18 keeps the clamping allowance small
+    net, solver = ncho
+    cells = 1024
+    inp = workloads.config0(net.species, 0, cells=cells, seed=2)
+    args = (t(inp["x"]), t(inp["T"]), t(inp["P"]), full(cells, 500.0), full(cells, 1e-10))
+    with CudaSolver(net, "cuda", drift=0.0) as strict:
+        y, _, dz, _, flag = strict.launch(*args, atol=1e-18)
+        counts = strict.fallback
+    yk, _, _, _, flag0 = solver.launch(*args, atol=1e-18)
+    passed = flag == 0
+    print(f"allowance 0: counts {counts}; default allowance flags {int(flag0.sum())}")
+    assert counts["drift"] > 0 and counts["recomputed"] == counts["drift"]
+    assert counts["recovered"] + counts["still_failed"] == counts["drift"]
+    assert int(flag0.sum()) == 0
+    assert torch.equal(y[passed], yk[passed])
+    assert not bool(dz[flag == 2].any())
+
+
+def test_temperature_out_of_range_is_refused(ncho, monkeypatch):
+    net, solver = ncho
+    cells = 256
+    x, T, P, h = starts(net, solver, cells, 2)
+    T = T.clone()
+    T[0], T[1] = 250.0, T_MAX + 0.5
+    T[2], T[3] = T_MIN, T_MAX  # the ends of the range are accepted
+    args = (x, T, P, full(cells, 500.0), full(cells, 1e-10))
+    refused = r"1 cells below 300 K, .*\(first: cell 0, T = 250 K\).*; 1 cells above 3000 K, .*cell 1, T = 3000.5 K\)"
+    with CudaSolver(net, "cuda") as fresh:
+        assert fresh.check()["cold"] == 0
+        with pytest.raises(ValueError, match=refused):
+            fresh(*args)
+        with pytest.raises(ValueError, match="1 cells below 300 K and 1 above 3000 K"):
+            fresh.check()
+    with monkeypatch.context() as m:
+        m.setattr(solver, "_refuse", lambda *_: None)
+        _, _, d, _ = solver(*args)
+    counts = solver.fallback
+    assert counts["cold"] == 1 and counts["hot"] == 1
+    assert not bool(d[:2].any()) and bool(d[2:4].all())
+
+
+def test_bad_inputs_are_refused_and_the_rest_untouched(ncho, monkeypatch):
+    net, solver = ncho
+    cells = 64
+    inp = workloads.config0(net.species, 0, cells=cells, seed=1)
+    xb, T, Pb = t(inp["x"]), t(inp["T"]), t(inp["P"])
+    dtb, hb = full(cells, 50.0), full(cells, 1e-10)
+    xb[3] = 0.0
+    xb[7, 0] = -1e-3
+    Pb[5] = 0.0
+    dtb[9] = -1.0
+    xb[11] *= 0.9
+    xb[13] *= 1.0 + 0.9 * X_SUM_TOL  # just inside the tolerance on either side: accepted
+    xb[15] *= 1.0 - 0.9 * X_SUM_TOL
+    bad = [3, 5, 7, 9, 11]
+    with pytest.raises(ValueError, match=r"5 cells of bad input: .*\(first: cell 3: sum\(x\) - 1 = -1 "):
+        solver(xb, T, Pb, dtb, hb)
+    with pytest.raises(ValueError, match=r"1 cells of bad input: .*\(first: cell 0: sum\(x\) - 1 = -0.1 "):
+        solver(xb[11:12], T[11:12], Pb[11:12], dtb[11:12], hb[11:12])
+    with monkeypatch.context() as m:
+        m.setattr(solver, "_refuse", lambda *_: None)
+        y, _, d, _ = solver(xb, T, Pb, dtb, hb)
+    counts = solver.fallback
+    good = torch.ones(cells, dtype=torch.bool, device="cuda")
+    good[bad] = False
+    yr, _, dr, _ = solver(xb[good], T[good], Pb[good], dtb[good], hb[good])
+    print(f"counts {counts}")
+    assert counts["bad_input"] == len(bad) and counts["nonfinite"] == 0 and counts["recomputed"] == 0
+    assert not bool(d[bad].any()) and bool(d[[13, 15]].all())
+    assert torch.equal(y[good], yr) and torch.equal(d[good], dr)
+
+
+def test_non_finite_and_overflowing_pressures_are_named(ncho):
+    net, solver = ncho
+    inp = workloads.config0(net.species, 0, cells=4, seed=1)
+    P = t(inp["P"])
+    P[1:] = t([np.inf, np.nan, 1e300])
+    named = r"^the call refused cells: 3 cells of bad input: .*P non-finite, not above 0 or so large that P / \(k_B T\)"
+    with pytest.raises(ValueError, match=rf"{named} overflows.*first: cell 1:"):
+        solver(t(inp["x"]), t(inp["T"]), P, full(4, 50.0), full(4, 1e-10))
+    assert solver.fallback["bad_input"] == 3
+
+
+def test_one_step_outside_the_range_is_refused(ncho, monkeypatch):
+    net, solver = ncho
+    inp = workloads.config0(net.species, 0, cells=4, seed=2)
+    cold, hot = np.nextafter(T_MIN, 0.0), np.nextafter(T_MAX, np.inf)
+    T = t([cold, hot, cold, hot])
+    dt, h = t([0.0, 0.0, 500.0, 500.0]), full(4, 1e-10)
+    x, P = t(inp["x"]), t(inp["P"])
+    refused = r"2 cells below 300 K, .*\(first: cell 0, T = 300 K\).*; 2 cells above 3000 K, .*\(first: cell 1, "
+    with pytest.raises(ValueError, match=refused):
+        solver(x, T, P, dt, h)
+    with monkeypatch.context() as m:
+        m.setattr(solver, "_refuse", lambda *_: None)
+        _, _, done, _ = solver(x, T, P, dt, h)
+    counts = solver.fallback
+    assert counts["cold"] == 2 and counts["hot"] == 2 and not bool(done.any())
+
+
+def test_non_finite_step_acts_as_dt(ncho):
+    net, solver = ncho
+    cells = 64
+    inp = workloads.config0(net.species, 0, cells=cells, seed=4)
+    x, T, P = t(inp["x"]), t(inp["T"]), t(inp["P"])
+    dt = full(cells, 50.0)
+    ref = solver(x, T, P, dt, dt.clone())
+    assert solver.fallback["nonfinite_h"] == 0
+    for q in (np.nan, -np.inf, np.inf):
+        out = solver(x, T, P, dt, full(cells, q))
+        assert same(out, ref), q
+        assert solver.fallback["nonfinite_h"] == cells
+
+
+def test_zero_dt_leaves_the_state(ncho):
+    net, solver = ncho
+    cells = 16
+    x, T, P, h = starts(net, solver, cells, 5)
+    r = solver.launch(x, T, P, full(cells, 0.0), h)
+    s = x / x.sum(1, keepdim=True)
+    assert bool(r.done.all()) and int(r.steps.sum()) == 0
+    assert torch.allclose(r.x, s, rtol=1e-12, atol=0.0)
+
+
+def test_launch_adds_to_the_total(ncho):
+    net, solver = ncho
+    x, T, P, h = starts(net, solver, 8, 2)
+    before = solver.total.clone()
+    solver.launch(x, T, P, full(8, 1.0), full(8, 1.0))
+    assert torch.equal(solver.total - before, solver.count.to(torch.int64))
+
+
+def test_call_and_launch_name_their_results(ncho):
+    net, solver = ncho
+    inp = workloads.config0(net.species, 0, cells=8, seed=2)
+    args = (t(inp["x"]), t(inp["T"]), t(inp["P"]), full(8, 50.0), full(8, 1e-10))
+    launched = solver.launch(*args, max_steps=2)  # few tries: the guard recomputes cells
+    assert isinstance(launched, LaunchResult) and solver.flag is launched.flag
+    assert bool((launched.flag != 0).any())
+    called = solver(*args, max_steps=2)
+    assert isinstance(called, CallResult)
+    assert all(torch.equal(getattr(called, k), getattr(launched, k)) for k in called._fields)
+    assert torch.equal(solver.flag, launched.flag)
+
+
+def test_flag_is_none_after_a_call_that_raised(ncho):
+    net, solver = ncho
+    x, T, P, h = starts(net, solver, 8, 2)
+    dt = full(8, 1.0)
+    bad = x.clone()
+    bad[3, 0] = float("nan")
+    solver(x, T, P, dt, h)
+    with pytest.raises(ValueError, match="refused cells"):
+        solver.launch(bad, T, P, dt, h)
+    assert solver.flag is None
+    solver(x, T, P, dt, h)
+    with pytest.raises(ValueError, match=r"^T: "):  # on the host, before any launch
+        solver(x[:5], T[:5].cpu(), P[:5], dt[:5], h[:5])
+    assert solver.flag is None
+
+
+def test_wrong_inputs_raise_before_any_launch(ncho):
+    net, solver = ncho
+    x, T, P, h = starts(net, solver, 8, 2)
+    dt = full(8, 50.0)
+    with pytest.raises(ValueError, match=r"^T: .* on cpu$"):
+        solver(x, T.cpu(), P, dt, h)
+    with pytest.raises(ValueError, match=r"^max_steps from 1"):
+        solver(x, T, P, dt, h, max_steps=0)
+    with pytest.raises(ValueError, match=r"^J: none or an empty"):
+        solver(x, T, P, dt, h, J=torch.zeros(8, 3, dtype=torch.float64, device="cuda"))
+    with CudaSolver(net, "cuda", photo=True) as lit:
+        with pytest.raises(ValueError, match="required"):
+            lit(x, T, P, dt, h)
+
+
+def test_closed_solver_refuses_a_call(ncho):
+    net, _ = ncho
+    with CudaSolver(net, "cuda") as solver:
+        assert solver.fallback["cold"] == 0
+    solver.close()  # twice does nothing
+    inp = workloads.config0(net.species, 0, cells=8, seed=2)
+    with pytest.raises(RuntimeError, match="closed"):
+        solver(t(inp["x"]), t(inp["T"]), t(inp["P"]), full(8, 1.0), full(8, 1.0))
+
+
+ALLOWANCE = 1e-6
+ATOMS = {"H": {"H": 1}, "H2": {"H": 2}, "OH": {"H": 1, "O": 1}, "H2O": {"H": 2, "O": 1}, "O": {"O": 1}, "He": {"He": 1}}
+DRIFTS = (0.5, 0.99, 1.01, 2.0)  # the largest relative change of an element share, in allowances
+
+
+def test_guard_kernel_flags_drift_beyond_the_allowance(read_network, gpu):
+    # the guard kernel alone, on states whose He share moved by r allowances; the oracle's check_cell agrees
+    net = read_network("synthetic_small")
+    x0 = np.array([{"H": 0.1, "H2": 0.6, "OH": 0.05, "H2O": 0.1, "O": 0.05, "He": 0.1}[q] for q in net.species])
+    atoms = np.array([[ATOMS[q].get(e, 0) for e in ("H", "O", "He")] for q in net.species], dtype=float)
+    share = (x0 @ atoms) / (x0 @ atoms).sum()
+    he, f = net.species.index("He"), share[2]
+    results = []
+    for r in DRIFTS:
+        d = r * ALLOWANCE / (1.0 - f - f * r * ALLOWANCE)
+        x = x0.copy()
+        x[he] *= 1.0 + d
+        moved = (x @ atoms) / (x @ atoms).sum() / share - 1.0
+        assert np.abs(moved).max() == pytest.approx(r * ALLOWANCE, rel=1e-6)
+        results.append(x)
+    cells = len(DRIFTS)
+    with CudaSolver(net, gpu, drift=ALLOWANCE) as solver:
+        x0d, xo = t([x0] * cells), t(results)
+        done = torch.ones(cells, dtype=torch.int32, device="cuda")
+        nok = full(cells, 1.0)
+        status, redo, lst = (torch.zeros(cells, dtype=torch.int32, device="cuda") for _ in range(3))
+        solver.count.zero_()
+        p = _hip.ptr
+        args = [p(solver._net), p(x0d), p(xo), p(done), p(nok), p(status), p(redo), p(lst), p(solver.count),
+                ctypes.c_int(cells), ctypes.c_double(1e-25)]  # fmt: skip
+        _hip.launch(solver._guard, 1, _kernels.GUARD_THREADS, 0, torch.cuda.current_stream().cuda_stream, args)
+        counts = solver.fallback
+    orc = oracle.Oracle(_tables.build(net, False), ALLOWANCE)
+    want = [int(orc.check_cell(x0, x, True, 1.0)[0] != 0) for x in results]
+    print(f"drift {DRIFTS} allowances: flagged {redo.tolist()}, oracle {want}; counts {counts}")
+    assert redo.tolist() == [int(r > 1.0) for r in DRIFTS] == want
+    assert counts["drift"] == counts["recomputed"] == sum(r > 1.0 for r in DRIFTS)
+

@@ -168,8 +168,8 @@ class Tables:
     def layout(self) -> dict[str, int]:
         """Offsets (doubles) in a cell's LDS region: LEAD [K0][NS], A21 [NT][K0], n [NS + 1], the rate
         coefficients KF, KR [NR] (kept for the call), dv [NDV + 1], then a union of what the rhs uses (v [NR + 1],
-        pieces [NPART]) with what the Jacobian and the factorization use after it (the trailing diagonal or the LU's
-        row buffer [NT], staging [NT][CHUNK]); G / RT [NS + 1] sits at its start while the rates are computed."""
+        pieces [NPART]) with what the Jacobian and the factorization use after it (the trailing diagonal [NT], staging
+        [NT][CHUNK]); G / RT [NS + 1] sits at its start while the rates are computed."""
         a21 = self.k0 * self.ns
         nvec = a21 + self.nt * self.k0
         kf = nvec + self.ns + 1
@@ -183,7 +183,7 @@ class Tables:
         size = max(part + self.npart, stage + self.nt * CHUNK, union + self.ns + 1)
         size += size & 1  # keep the next cell's region 16-byte aligned
         return {"A21": a21, "NVEC": nvec, "KF": kf, "KR": kr, "DV": dv, "UNION": union, "V": v, "PART": part,
-                "DIAG": diag, "STAGE": stage, "ROW": diag, "SIZE": size}
+                "DIAG": diag, "STAGE": stage, "SIZE": size}
 
 
 def build(net: VulcanNetwork, photo: bool = False) -> Tables:

@@ -22,6 +22,7 @@ constexpr double TROE_D = 0.14;
 constexpr int REFUSED_COLD = 1;
 constexpr int REFUSED_BAD_INPUT = 2;
 constexpr int REFUSED_HOT = 3;
+constexpr int STREAM_BATCH = 8;  // rounds of a lane stream whose words are loaded together
 
 // Orders the LDS accesses of the wavefront's lanes: the wave executes in lockstep and LDS serves one wave's accesses
 // in issue order, so only the compiler must not move loads and stores across this point.
@@ -76,6 +77,13 @@ __device__ __forceinline__ int opaque_lane(int lane) {
   return lane;
 }
 
+// The lane index as an opaque value that the compiler must take after x is computed: a step's lane mask then
+// cannot be computed ahead of the step whose value x it follows.
+__device__ __forceinline__ int lane_after(int lane, double x) {
+  asm volatile("" : "+v"(lane) : "v"(x));
+  return lane;
+}
+
 // The table pointer p, made opaque to the compiler where it is used, so that the loads and decodes from the
 // per-network tables are not hoisted out of the try loop and held in registers across it.
 template <typename P>
@@ -83,6 +91,11 @@ __device__ __forceinline__ P opaque(P p) {
   asm volatile("" : "+s"(p));
   return p;
 }
+
+// a and b materialised here, in this order with the other volatile statements (an empty asm that reads and writes
+// them): it keeps a step's values from being computed early or late, which would stretch their live ranges.
+__device__ __forceinline__ void pin(double& a, double& b) { asm volatile("" : "+v"(a), "+v"(b)); }
+__device__ __forceinline__ void pin(double& a) { asm volatile("" : "+v"(a)); }
 
 __device__ __forceinline__ bool wave_all(bool p) { return __ballot(!p) == 0ull; }
 
@@ -218,8 +231,11 @@ __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& 
   const double* kr = ws + L_KR;
   double* V = ws + L_V;
   double* DV = ws + L_DV;
-#pragma unroll 1
-  for (int r = lane; r < NR; r += WAVE) {
+  // unrolled, so that the reaction words of every round are loaded ahead of their use
+#pragma unroll
+  for (int q = 0; q < RROUNDS; q++) {
+    const int r = q * WAVE + lane;
+    if (r >= NR) break;
     const u64 w = rxe[r];
     const double n0 = n[species_of(w, 0)], n1 = n[species_of(w, 1)], n2 = n[species_of(w, 2)];
     const double n3 = n[species_of(w, 3)], n4 = n[species_of(w, 4)], n5 = n[species_of(w, 5)];
@@ -246,13 +262,21 @@ __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& 
   wave_sync();
   double* part = ws + L_PART;
   double acc = 0.0;
-#pragma unroll 2
-  for (int q = 0; q < SV_ROUNDS; q++) {
-    const u32 w = svw[q * WAVE + lane];
-    acc = fma(coefficient(w, SV_NEG, SV_DBL), V[w & SV_R_MASK], acc);
-    if ((w >> SV_END) & 1u) {
-      part[w >> SV_PIECE] = acc;
-      acc = 0.0;
+#pragma unroll 1
+  for (int q0 = 0; q0 < SV_ROUNDS; q0 += STREAM_BATCH) {  // as jac_stream: a batch of rounds' words at a time
+    u32 w[STREAM_BATCH];
+    double d[STREAM_BATCH];
+#pragma unroll
+    for (int i = 0; i < STREAM_BATCH; i++) w[i] = q0 + i < SV_ROUNDS ? svw[(q0 + i) * WAVE + lane] : (u32)NR;
+#pragma unroll
+    for (int i = 0; i < STREAM_BATCH; i++) d[i] = V[w[i] & SV_R_MASK];
+#pragma unroll
+    for (int i = 0; i < STREAM_BATCH; i++) {
+      acc = fma(coefficient(w[i], SV_NEG, SV_DBL), d[i], acc);
+      if ((w[i] >> SV_END) & 1u) {
+        part[w[i] >> SV_PIECE] = acc;
+        acc = 0.0;
+      }
     }
   }
   wave_sync();
@@ -274,17 +298,28 @@ __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& 
   return ntot;
 }
 
-// One Jacobian stream: each lane sums whole entries of -Jn from dv terms and writes each at its LDS offset.
+// One Jacobian stream: each lane sums whole entries of -Jn from dv terms and writes each at its LDS offset. The
+// words of STREAM_BATCH rounds are loaded together, ahead of their terms, so that their latency is paid once per
+// batch; rounds past the end take the padding word (the zero term, no end).
+
 __device__ __forceinline__ void jac_stream(const u32* __restrict__ words, int rounds, const double* DV, double* ws,
                                            int lane) {
   double acc = 0.0;
-#pragma unroll 2
-  for (int q = 0; q < rounds; q++) {
-    const u32 w = words[q * WAVE + lane];
-    acc = fma(coefficient(w, T_NEG, T_DBL), DV[w & T_IDX_MASK], acc);
-    if ((w >> T_END) & 1u) {
-      ws[w >> T_DEST] = acc;
-      acc = 0.0;
+#pragma unroll 1
+  for (int q0 = 0; q0 < rounds; q0 += STREAM_BATCH) {
+    u32 w[STREAM_BATCH];
+    double d[STREAM_BATCH];
+#pragma unroll
+    for (int i = 0; i < STREAM_BATCH; i++) w[i] = q0 + i < rounds ? words[(q0 + i) * WAVE + lane] : (u32)NDV;
+#pragma unroll
+    for (int i = 0; i < STREAM_BATCH; i++) d[i] = DV[w[i] & T_IDX_MASK];
+#pragma unroll
+    for (int i = 0; i < STREAM_BATCH; i++) {
+      acc = fma(coefficient(w[i], T_NEG, T_DBL), d[i], acc);
+      if ((w[i] >> T_END) & 1u) {
+        ws[w[i] >> T_DEST] = acc;
+        acc = 0.0;
+      }
     }
   }
 }
@@ -300,12 +335,16 @@ __device__ __forceinline__ void jacobian(double (&c)[NT], double Nl, double Nt, 
   wave_sync();
   jac_stream(opaque(net)->diagw, DIAG_ROUNDS, DV, ws, lane);
   jac_stream(opaque(net)->leadw, LEAD_ROUNDS, DV, ws, lane);
+  wave_sync();
+  const double cd = lane < NT ? ws[L_DIAG + lane] : 0.0;
   double* stage = ws + L_STAGE;
 #pragma unroll
   for (int ch = 0; ch < NCH; ch++) {
     if (lane < NT) {
 #pragma unroll
       for (int j = 0; j < CHUNK; j++) stage[lane * CHUNK + j] = 0.0;
+      // the chunk streams hold no diagonal entry: a trailing row's own diagonal goes in by its lane
+      if (lane / CHUNK == ch) stage[lane * CHUNK + lane % CHUNK] = cd;
     }
     wave_sync();
     jac_stream(opaque(net)->chw + CH_OFF[ch] * WAVE, CH_ROUNDS[ch], DV, ws, lane);
@@ -314,25 +353,34 @@ __device__ __forceinline__ void jacobian(double (&c)[NT], double Nl, double Nt, 
     for (int j = 0; j < CHUNK; j++)
       if (ch * CHUNK + j < NT) c[ch * CHUNK + j] = lane < NT ? stage[lane * CHUNK + j] : 0.0;
   }
-  const double cd = lane < NT ? ws[L_DIAG + lane] : 0.0;
-#pragma unroll
-  for (int j = 0; j < NT; j++) c[j] += j == lane ? cd : 0.0;  // the staging held 0 on the diagonal
-  // the state N into the exchange vector (n's place, free until the next rhs), then w
+  // -Jn N: the lead columns through the exchange vector (n's place, free until the next rhs), the trailing
+  // columns of the trailing rows with N_j taken from lane j
   double* xv = ws + L_NVEC;
   wave_sync();
   if (lane < K0) xv[lane] = Nl;
   if (lane < NT) xv[K0 + lane] = Nt;
   wave_sync();
   double ml = 0.0, mt = 0.0;
-  if (lane < K0)
-    for (int j = 0; j < NS; j++) ml = fma(ws[lane * NS + j], xv[j], ml);
-  if (lane < NT) {
-    for (int k = 0; k < K0; k++) mt = fma(ws[L_A21 + lane * K0 + k], xv[k], mt);
+  if (K0 > 0) {
+#pragma unroll 1
+    for (int j0 = 0; j0 < NS; j0 += STREAM_BATCH) {  // the lead rows, a batch of columns at a time
+      double a[STREAM_BATCH], nv[STREAM_BATCH];
 #pragma unroll
-    for (int j = 0; j < NT; j++) mt = fma(c[j], xv[K0 + j], mt);
+      for (int i = 0; i < STREAM_BATCH; i++) {
+        const int j = j0 + i;
+        a[i] = lane < K0 && j < NS ? ws[lane * NS + j] : 0.0;
+        nv[i] = j < NS ? xv[j] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < STREAM_BATCH; i++) ml = fma(a[i], nv[i], ml);
+    }
+#pragma unroll 1
+    for (int k = 0; k < K0; k++) mt = fma(lane < NT ? ws[L_A21 + lane * K0 + k] : 0.0, xv[k], mt);
   }
+#pragma unroll
+  for (int j = 0; j < NT; j++) mt = fma(c[j], readlane(Nt, j), mt);
   ul = u0l + ml / ntot;
-  ut = u0t + mt / ntot;
+  ut = lane < NT ? u0t + mt / ntot : 0.0;
   wave_sync();
 }
 
@@ -377,34 +425,31 @@ __device__ __forceinline__ void factor(double (&c)[NT], double& dl, double& dt, 
       for (int j = 0; j < NT; j++) c[j] = fma(-a, uk[j], c[j]);
     }
   }
-  // the trailing block: at pivot k, lane k publishes its row's tail in the row buffer (LDS, the diagonal's place,
-  // free after the Jacobian), which every lane then reads as a broadcast
-  double* row = ws + L_ROW;
-  dt = 1.0;
+  // the trailing block: at pivot k every lane takes row k's tail from lane k by v_readlane, as wave-uniform values
+  // in scalar registers (no LDS round trip, no wait)
+  // 1 / U_kk goes through LDS (the union region is free here): a select per step into a register would have the
+  // compiler hold every step's reciprocal to the end
+  double* dinv = ws + L_UNION;
 #pragma unroll
   for (int k = 0; k < NT; k++) {
-    if (lane == k) {
+    const double dk = 1.0 / readlane(c[k], k);
+    if (lane == 0) dinv[k] = dk;
+    // column k is scaled by 1 / U_kk off the diagonal: below the pivot it becomes the multiplier L_ik, above it
+    // U_ik / U_kk, the unit upper factor the backward solve uses (U = U' diag(U_kk))
+    if (lane != k) c[k] *= dk;
+    const double l = lane > k ? c[k] : 0.0;
 #pragma unroll
-      for (int j = k; j < NT; j++) row[j] = c[j];
-    }
-    wave_sync();
-    const double dk = 1.0 / row[k];
-    if (lane == k) dt = dk;
-    // below the pivot c[k] becomes the multiplier; on and above it stays (times 1, exactly): a product, not a
-    // select, so that the compiler keeps one copy of the row
-    const bool below = opaque_lane(lane) > k;
-    c[k] *= below ? dk : 1.0;
-    const double l = below ? c[k] : 0.0;
-#pragma unroll
-    for (int j = k + 1; j < NT; j++) c[j] = fma(-l, row[j], c[j]);
-    wave_sync();
+    for (int j = k + 1; j < NT; j++) c[j] = fma(-l, readlane(c[j], k), c[j]);
   }
+  wave_sync();
+  dt = lane < NT ? dinv[lane] : 1.0;
+  wave_sync();
 }
 
 // y <- M0^-1 y for NV vectors (lead / trailing parts), with factor's factors.
 template <int NV>
 __device__ __forceinline__ void solve(double (&yl)[NV], double (&yt)[NV], const double (&c)[NT], double dl, double dt,
-                                      const double* ws, int lane) {
+                                      double* ws, int lane) {
   const double* A21 = ws + L_A21;
 #pragma unroll 1
   for (int k = 0; k < K0; k++) {  // unit L, lead columns
@@ -417,35 +462,60 @@ __device__ __forceinline__ void solve(double (&yl)[NV], double (&yt)[NV], const 
       yt[v] = fma(-tk, yk, yt[v]);
     }
   }
-  // the trailing block's solves mask the update, not the factor entry: a masked copy of c[k] would depend on the
-  // factors alone, and the compiler would compute all of them ahead of the chain and hold them in registers
+  // the trailing block's solves: at step k every lane takes lane k's final value by v_readlane and is updated with
+  // its factor entry in column k, unmasked; lane k's value is recorded before its own update spoils it (and the
+  // lanes already recorded are spoilt freely), so the chain from step to step is a readlane and an fma. pin keeps
+  // each step's record in its step: left free, the compiler sinks the records to the end and holds every step's
+  // value until then
+  double rec[NV];
 #pragma unroll
   for (int k = 0; k < NT; k++) {  // unit L, trailing block
-    const bool below = opaque_lane(lane) > k;
+    const bool own = lane_after(lane, yt[0]) == k;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       const double yk = readlane(yt[v], k);
-      yt[v] = below ? fma(-c[k], yk, yt[v]) : yt[v];
+      rec[v] = own ? yt[v] : (k == 0 ? 0.0 : rec[v]);
+      if (k + 1 < NT) yt[v] = fma(-c[k], yk, yt[v]);
+      pin(rec[v], yt[v]);
     }
   }
 #pragma unroll
-  for (int k = NT - 1; k >= 0; k--) {  // U, trailing block: lane k's value is final here, and it scales it
-    const bool above = opaque_lane(lane) < k;
+  for (int k = NT - 1; k >= 0; k--) {  // unit U' = U diag(1 / U_kk), trailing block
+    const bool own = lane_after(lane, rec[0]) == k;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      const double zk = readlane(yt[v] * dt, k);
-      yt[v] = above ? fma(-c[k], zk, yt[v]) : yt[v];
+      const double zk = readlane(rec[v], k);
+      yt[v] = own ? rec[v] : (k == NT - 1 ? 0.0 : yt[v]);
+      if (k > 0) rec[v] = fma(-c[k], zk, rec[v]);
+      pin(rec[v], yt[v]);
     }
   }
 #pragma unroll
   for (int v = 0; v < NV; v++) yt[v] *= dt;
   if (K0 > 0) {
+    // U of the lead rows, trailing columns: the trailing solution through LDS (the union region is free between
+    // the right-hand sides), a batch of columns at a time
+    double* xv = ws + L_UNION;
+    wave_sync();
 #pragma unroll
-    for (int t = 0; t < NT; t++) {  // U of the lead rows, trailing columns
-      const double a = lane < K0 ? ws[lane * NS + K0 + t] : 0.0;
+    for (int v = 0; v < NV; v++)
+      if (lane < NT) xv[v * NT + lane] = yt[v];
+    wave_sync();
+    if (lane < K0) {
+#pragma unroll 1
+      for (int t0 = 0; t0 < NT; t0 += STREAM_BATCH) {
+        double a[STREAM_BATCH];
 #pragma unroll
-      for (int v = 0; v < NV; v++) yl[v] = fma(-a, readlane(yt[v], t), yl[v]);
+        for (int i = 0; i < STREAM_BATCH; i++) a[i] = t0 + i < NT ? ws[lane * NS + K0 + t0 + i] : 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+#pragma unroll
+          for (int i = 0; i < STREAM_BATCH; i++)
+            if (t0 + i < NT) yl[v] = fma(-a[i], xv[v * NT + t0 + i], yl[v]);
+        }
+      }
     }
+    wave_sync();
 #pragma unroll 1
     for (int k = K0 - 1; k >= 0; k--) {  // U, lead columns
       const double uk = lane < k ? ws[lane * NS + k] : 0.0;

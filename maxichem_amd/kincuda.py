@@ -149,11 +149,17 @@ class CudaSolver(Solver):
             xo, ho = torch.empty_like(x), h.clone()
             done, redo, lst, status = (torch.empty(B, dtype=torch.int32, device=self.dev) for _ in range(4))
             nok = torch.empty(B, dtype=torch.float64, device=self.dev)
+            # the queue's order: the previous call's costliest cells first when it had as many cells (a call
+            # usually follows one over the same cells); results do not depend on it
+            prev = self.tries
+            order = None
+            if prev is not None and prev.shape[0] == B:
+                order = torch.argsort(prev, descending=True).to(torch.int32)
             self.tries = torch.zeros(B, dtype=torch.float64, device=self.dev)
             self.count.zero_()
             stream = torch.cuda.current_stream(self.dev).cuda_stream
             self._launch_all(x, T, P, dt, h, xo, ho, done, nok, status, redo, lst, jx, B, rtol, atol, max_steps,
-                             stream, None)  # fmt: skip
+                             stream, order)  # fmt: skip
             self.total += self.count
             self._refuse(x, T, P, dt, jx, status)
         self.flag = redo

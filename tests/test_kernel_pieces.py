@@ -58,6 +58,7 @@ def probe(net, photo, x, T, P, h, J):
     got["lu"] = o[b + 2 * ns + ns * ns : b + 2 * ns + 2 * ns * ns].reshape(ns, ns)  # elimination order
     got["dinv"] = o[b + 2 * ns + 2 * ns * ns : b + 3 * ns + 2 * ns * ns]
     got["perm"] = perm
+    got["k0"] = tab.k0
     v = b + 3 * ns + 2 * ns * ns
     for k, name in enumerate(("m0f", "K1", "K2", "K3", "K4")):
         got[name] = net_order(o[v + k * ns : v + (k + 1) * ns])
@@ -93,10 +94,14 @@ def test_one_try_matches_numpy(read_network, stem, photo):
     for k in range(n):
         A[k + 1 :, k] /= A[k, k]
         A[k + 1 :, k + 1 :] -= np.outer(A[k + 1 :, k], A[k, k + 1 :])
+    k0 = int(got["k0"])
+    U = A.copy()
+    for k in range(k0, n):  # the trailing block keeps its U columns scaled: U_ik / U_kk above the diagonal
+        A[k0:k, k] /= U[k, k]
     bad = np.argwhere(~np.isclose(got["lu"], A, rtol=1e-9, atol=1e-12 * np.abs(A).max()))
     assert len(bad) == 0, f"LU factors differ at (row, column) {[tuple(map(int, q)) for q in bad[:40]]}, " + str(
         [(float(got["lu"][i, j]), float(A[i, j])) for i, j in bad[:8]])
-    np.testing.assert_allclose(got["dinv"], 1.0 / np.diag(A), rtol=1e-12, err_msg="1 / U_ii")
+    np.testing.assert_allclose(got["dinv"], 1.0 / np.diag(U), rtol=1e-12, err_msg="1 / U_ii")
     m0f = np.linalg.solve(W0, F)
     np.testing.assert_allclose(got["m0f"], m0f, rtol=1e-8, atol=1e-10 * np.abs(m0f).max())
     W = np.eye(len(x)) / (0.5 * h) - JN

@@ -37,6 +37,11 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--network", default="NCHO", choices=("NCHO", "SNCHO"))
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="leave out the CPU baseline (profiling runs)")
+    ap.add_argument("--form", default="both", choices=("plain", "learned", "both"),
+                    help="plain FP64, the learned warm start, or both on the same start (value: the learned form)")
+    ap.add_argument("--train-cells", type=int, default=16384, help="cells per training sequence")
+    ap.add_argument("--train-sequences", type=int, default=6, help="training sequences (seeds 101..)")
+    ap.add_argument("--train-epochs", type=float, default=4.0)
     return ap.parse_args()
 
 
@@ -82,6 +87,64 @@ def cpu_baseline(tab, start: dict, first, cells: int, seconds: float) -> dict:
     }  # fmt: skip
 
 
+def timed(solver, x, T, P, dt, h, J, steps: int, flops_of) -> dict:
+    """steps calls from (x, h), each from the previous one's result: wall time, the call kernel's events and flops."""
+    import numpy as np
+    import torch
+
+    solver.timing = []
+    tries, first = [], None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = solver(x, T, P, dt, h, J=J)
+        x, h = r.x, r.h
+        tries.append(solver.tries)
+        if first is None:
+            first = r
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    events, solver.timing = solver.timing, None
+    kernel_ms = np.array([a.elapsed_time(b) for a, b in events])
+    tries = [t.cpu().numpy() for t in tries]
+    per_launch = np.array([flops_of(t) for t in tries])
+    return {"wall": wall, "kernel_ms": kernel_ms, "tries": tries, "first": first, "last": r,
+            "achieved": float(per_launch.sum() / (kernel_ms.sum() * 1e-3) / 1e12)}  # fmt: skip
+
+
+def train_surrogate(net, plain, args, log_fn):
+    """The learned form's surrogate, trained on calls of the plain solver over sequences drawn as the workload but
+    from other seeds (101, 102, ...): a dark relaxation and calls of dt = 300 s, and cold calls of dt log-uniform
+    1e1-1e6 s."""
+    import numpy as np
+
+    from maxichem_amd import workloads
+    from maxichem_amd.learned import Surrogate, generate
+
+    nj = len(net.photo)
+    seqs = []
+    for k in range(args.train_sequences):
+        if k % 3 == 2:
+            d = workloads.config3(net.species, nj, cells=args.train_cells, seed=101 + k)
+            seqs.append({"x": d["x"], "T": d["T"], "P": d["P"], "h": d["h"], "calls": [(d["dt"], d["J"])]})
+        else:
+            d = workloads.config1(net.species, nj, cells=args.train_cells, seed=101 + k)
+            dark = np.zeros_like(d["J"])
+            seqs.append({"x": d["x"], "T": d["T"], "P": d["P"], "h": d["h"],
+                         "calls": [(d["dt_relax"], dark)] + [(d["dt"], d["J"])] * 8})  # fmt: skip
+    t0 = time.perf_counter()
+    data = generate(plain, seqs)
+    n = int(data["x0"].shape[0])
+    log_fn(f"surrogate data: {n} calls in {time.perf_counter() - t0:.1f} s")
+    sur = Surrogate(net, plain.dev, seed=0)
+    sur.standardise(data)
+    t0 = time.perf_counter()
+    losses = sur.fit(data, epochs=args.train_epochs, batch=8192, lr=2e-3)
+    log_fn(f"surrogate training: {time.perf_counter() - t0:.1f} s, epoch losses {[round(v, 4) for v in losses]}, "
+           f"mean abs errors {sur.evaluate(data)}")  # fmt: skip
+    return sur, n
+
+
 def main() -> int:
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -121,65 +184,82 @@ def main() -> int:
         x, h = r.x, r.h
         log(f"warmup call {w + 1}/{args.warmup} done")
     start = {"x": x, "T": T, "P": P, "dt": dt, "h": h, "J": J}
-    start = {k: v.cpu().numpy() for k, v in start.items()}
+    start_np = {k: v.cpu().numpy() for k, v in start.items()}
+    work = flops.work(solver.tab)
 
-    solver.timing = []
-    tries, first = [], None
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        r = solver(x, T, P, dt, h, J=J)
-        x, h = r.x, r.h
-        tries.append(solver.tries)
-        if first is None:
-            first = r
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    events, solver.timing = solver.timing, None
-    kernel_ms = [a.elapsed_time(b) for a, b in events]
-    tries = [t.cpu().numpy() for t in tries]
-    done = int(r.done.sum())
-    log(f"{args.steps} timed calls: {wall:.3f} s; call kernel {np.mean(kernel_ms):.3f} ms average; last call done "
-        f"{done}/{args.cells}; mean tries per cell-call {np.mean([t.mean() for t in tries]):.2f}")  # fmt: skip
+    def flops_of(t):
+        return flops.call_flops(solver.tab, t)
 
-    per_launch = np.array([flops.call_flops(solver.tab, t) for t in tries])
-    achieved = float(per_launch.sum() / (np.sum(kernel_ms) * 1e-3) / 1e12)
-    w = flops.work(solver.tab)
+    runs = {}
+    if args.form in ("plain", "both"):
+        runs["plain"] = timed(solver, x, T, P, dt, h, J, args.steps, flops_of)
+    if args.form in ("learned", "both"):
+        from maxichem_amd.learned import LearnedSolver
+
+        sur, ndata = train_surrogate(net, solver, args, log)
+        learned = LearnedSolver(net, dev, sur, photo=True)
+        learned(x, T, P, dt, h, J=J)  # warm-up of the learned path (its first call from the same start)
+        runs["learned"] = timed(learned, x, T, P, dt, h, J, args.steps, flops_of)
+    for name, rr in runs.items():
+        log(f"{name}: {args.steps} timed calls {rr['wall']:.3f} s; call kernel {rr['kernel_ms'].mean():.3f} ms "
+            f"average; mean tries per cell-call {np.mean([t.mean() for t in rr['tries']]):.2f}; last call done "
+            f"{int(rr['last'].done.sum())}/{args.cells}")  # fmt: skip
+
+    headline = "learned" if "learned" in runs else "plain"
+    main_run = runs[headline]
     line = {
         "metric": "cells/s per call (NCHO, 64K cells): learned+FP32-LU vs plain FP64 at same error",
-        "value": args.cells * args.steps / wall,
+        "value": args.cells * args.steps / main_run["wall"],
         "unit": "cells/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": main_run["wall"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.network} {args.cells} cells, plain FP64 form: dark relaxation dt = 1e8 s (untimed), "
-                        "then calls of dt = 300 s carrying x and h, J log-uniform 1e-9..1e-3 s^-1 on the lowest-P "
-                        "quarter",
+            "workload": f"{args.network} {args.cells} cells: dark relaxation dt = 1e8 s (untimed), then calls of "
+                        "dt = 300 s carrying x and h, J log-uniform 1e-9..1e-3 s^-1 on the lowest-P quarter",
             "network": f"{'stand-in ' if getattr(net, 'standin', True) else ''}{args.network}: "
                        f"{len(net.species)} species, {len(net.reac)} thermal, {nj} photolysis",
-            "cells": args.cells, "seed": 1, "form": "plain FP64", "rtol": 1e-3, "atol": 1e-25,
+            "cells": args.cells, "seed": 1, "form": headline, "rtol": 1e-3, "atol": 1e-25,
         },
+        "forms": {},
         "roofline": {
             "bound": "mfma", "compute": "FP64 (vector FMA; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
-            "achieved": achieved, "peak": FP64_PEAK, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK, "traffic": None,
-            "kernel": "mxc_call", "kernel_ms": float(np.mean(kernel_ms)),
-            "flop_per_try": w.per_try, "flop_rates_per_cell": w.rates,
-            "tries_per_launch": float(np.mean([t.sum() for t in tries])),
+            "achieved": main_run["achieved"], "peak": FP64_PEAK, "unit": "TFLOP/s",
+            "frac": main_run["achieved"] / FP64_PEAK, "traffic": None,
+            "kernel": "mxc_call", "kernel_ms": float(main_run["kernel_ms"].mean()),
+            "flop_per_try": work.per_try, "flop_rates_per_cell": work.rates,
+            "tries_per_launch": float(np.mean([t.sum() for t in main_run["tries"]])),
         },  # fmt: skip
         "cpu_baseline": None,
     }
+    for name, rr in runs.items():
+        line["forms"][name] = {
+            "cells_per_s": args.cells * args.steps / rr["wall"], "ms_per_call": rr["wall"] / args.steps * 1e3,
+            "kernel_ms": float(rr["kernel_ms"].mean()), "fp64_tflops": rr["achieved"],
+            "frac_fp64_peak": rr["achieved"] / FP64_PEAK,
+            "tries_per_cell_call": float(np.mean([t.mean() for t in rr["tries"]])),
+        }  # fmt: skip
+    if "plain" in runs and "learned" in runs:
+        a, b = runs["plain"]["first"], runs["learned"]["first"]
+        big = a.x > 1e-20
+        rel = float((b.x[big] / a.x[big] - 1.0).abs().max())
+        sh = lambda x: (x @ solver._tables["atoms"]) / (x @ solver._tables["atoms"]).sum(1, keepdim=True)  # noqa: E731
+        line["forms"]["learned"]["vs_plain_first_call"] = {
+            "max_rel_species_above_1e-20": rel, "max_share_diff": float((sh(a.x) - sh(b.x)).abs().max()),
+            "done_match": bool(torch.equal(a.done, b.done)), "training_calls": ndata,
+        }  # fmt: skip
+        line["forms"]["learned"]["speedup_vs_plain"] = runs["plain"]["wall"] / runs["learned"]["wall"]
     if not args.no_cpu:
+        first = runs["plain"]["first"] if "plain" in runs else main_run["first"]
         first_np = {"x": first.x.cpu().numpy(), "done": first.done.cpu().numpy()}
         log("CPU baseline: the oracle on a sample of the first timed call")
-        line["cpu_baseline"] = cpu_baseline(solver.tab, start, first_np, args.cells, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(solver.tab, start_np, first_np, args.cells, args.cpu_seconds)
     solver.close()
     print(json.dumps(line), flush=True)
     return 0

@@ -89,6 +89,14 @@ def header(tab: Tables, cpw: int, drift: float) -> str:
     return "\n".join(lines) + "\n"
 
 
+MLP_FILE = "mlp.hip"
+
+
+def mlp_source() -> str:
+    """The learned warm start's kernels (kernels/mlp.hip): not specialised to a network."""
+    return (KERNEL_DIR / MLP_FILE).read_text()
+
+
 def source(tab: Tables, cpw: int, drift: float) -> str:
     """The whole translation unit hiprtc compiles: the header, then the kernel files in order."""
     parts = [header(tab, cpw, drift)]

@@ -149,29 +149,37 @@ class CudaSolver(Solver):
             xo, ho = torch.empty_like(x), h.clone()
             done, redo, lst, status = (torch.empty(B, dtype=torch.int32, device=self.dev) for _ in range(4))
             nok = torch.empty(B, dtype=torch.float64, device=self.dev)
-            # the queue's order: the previous call's costliest cells first when it had as many cells (a call
-            # usually follows one over the same cells); results do not depend on it
-            prev = self.tries
-            order = None
-            if prev is not None and prev.shape[0] == B:
-                order = torch.argsort(prev, descending=True).to(torch.int32)
+            order, xprop = self._plan(x, T, P, dt, ho, jx, B)
             self.tries = torch.zeros(B, dtype=torch.float64, device=self.dev)
             self.count.zero_()
             stream = torch.cuda.current_stream(self.dev).cuda_stream
             self._launch_all(x, T, P, dt, h, xo, ho, done, nok, status, redo, lst, jx, B, rtol, atol, max_steps,
-                             stream, order)  # fmt: skip
+                             stream, order, xprop)  # fmt: skip
             self.total += self.count
             self._refuse(x, T, P, dt, jx, status)
         self.flag = redo
         return LaunchResult(xo, ho, done.bool(), nok, redo)
 
+    def _plan(self, x, T, P, dt, h_start, jx, B):
+        """The queue's order and proposals of a call; h_start (the first step sizes) may be changed in place.
+
+        The plain form queues the previous call's costliest cells first when it had as many cells (a call usually
+        follows one over the same cells) and proposes nothing. Results do not depend on the order."""
+        prev = self.tries
+        order = None
+        if prev is not None and prev.shape[0] == B:
+            order = torch.argsort(prev, descending=True).to(torch.int32)
+        return order, None
+
     def _launch_all(self, x, T, P, dt, h, xo, ho, done, nok, status, redo, lst, jx, B, rtol, atol, max_steps,
-                    stream, order) -> None:  # fmt: skip
+                    stream, order, xprop=None) -> None:  # fmt: skip
+        """The call, guard and recovery kernels on stream. ho holds each cell's first step size on entry (the
+        recovery starts from h); xprop, when given, the proposed end states [B, NS] (NaN first entry: none)."""
         p = _hip.ptr
         net = p(self._net)
         call_args = [net, p(x), p(T), p(P), p(dt), p(ho), p(xo), p(done), p(nok), p(status), p(self.count),
                      p(self.tries), p(order), p(jx), ctypes.c_int(B), ctypes.c_double(rtol), ctypes.c_double(atol),
-                     ctypes.c_int(max_steps)]  # fmt: skip
+                     ctypes.c_int(max_steps), p(xprop)]  # fmt: skip
         grid = max(1, min(-(-B // self.cpw), self.max_blocks))
         if self.timing is not None:
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))

@@ -1,0 +1,189 @@
+"""The learned warm start's kernels (GPU): the MFMA product's lane layout, the forward pass and every layer's gradient
+against a numpy restatement that rounds to BF16 where the kernels do, and Adam reducing a regression loss."""
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+from scipy.special import erf
+
+from maxichem_amd import _hip
+from maxichem_amd.learned import MLP
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    """float32 -> nearest-even BF16, returned as float64 (finite values)."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def bf16_bits(a):
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16).view(np.int16)
+
+
+def gelu(z):
+    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+
+
+def gelu_grad(z):
+    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
+
+
+def dev(a, dtype):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dtype=dtype, device="cuda")
+
+
+@pytest.fixture(scope="module")
+def gpu_ok():
+    if not torch.cuda.is_available():
+        pytest.skip("this test needs a GPU")
+
+
+def test_mfma_product_matches_numpy(gpu_ok):
+    # C = A B on one 64 x 64 tile with inner 64 (two staging steps), and a transposed B by strides
+    rng = np.random.default_rng(0)
+    A, B = rng.standard_normal((64, 64)), rng.standard_normal((64, 128))
+    net = MLP(8, 8, "cuda", width=64, depth=1)
+    Ad, Bd = dev(bf16_bits(A), torch.int16), dev(bf16_bits(B), torch.int16)
+    C = torch.zeros(64, 128, dtype=torch.float32, device="cuda")
+    net._gemm("mlp_gemm_f32", 64, 128, 64, Ad, 64, 1, Bd, 128, 1, [_hip.ptr(C), ctypes.c_long(128), ctypes.c_float(1.0)])
+    want = bf16(A) @ bf16(B)
+    np.testing.assert_allclose(C.cpu().numpy(), want, rtol=1e-5, atol=1e-5)
+    Bt = np.ascontiguousarray(B.T)  # [128, 64]: (k, n) at n 64 + k
+    Ct = torch.zeros(64, 128, dtype=torch.float32, device="cuda")
+    net._gemm("mlp_gemm_f32", 64, 128, 64, Ad, 64, 1, dev(bf16_bits(Bt), torch.int16), 1, 64,
+              [_hip.ptr(Ct), ctypes.c_long(128), ctypes.c_float(1.0)])  # fmt: skip
+    np.testing.assert_allclose(Ct.cpu().numpy(), want, rtol=1e-5, atol=1e-5)
+
+
+def reference(net, X, T, w, rows):
+    """Forward and backward in float64 with the kernels' BF16 roundings: the activations between layers, the weights
+    and the gradients fed to the products."""
+    Ws = [bf16(ly.w.cpu().numpy()) for ly in net.layers]
+    bs = [ly.b.cpu().numpy().astype(np.float64) for ly in net.layers]
+    a = [bf16(X)]
+    z = []
+    for l in range(net.depth):
+        zl = a[-1] @ Ws[l].T + bs[l]
+        z.append(zl)
+        a.append(bf16(gelu(zl)))
+    y = a[-1] @ Ws[-1].T + bs[-1]
+    cols = net.n_out
+    d = np.zeros_like(y)
+    d[:rows, :cols] = 2.0 * w[None, :] * (y[:rows, :cols] - T) / (rows * cols)
+    dz = bf16(d)
+    gw, gb = [None] * (net.depth + 1), [None] * (net.depth + 1)
+    for l in range(net.depth, -1, -1):
+        gw[l] = dz.T @ a[l]
+        gb[l] = dz.sum(0)
+        if l > 0:
+            dz = bf16((dz @ Ws[l]) * gelu_grad(z[l - 1]))
+    return y, gw, gb
+
+
+def test_forward_and_gradients_match_numpy(gpu_ok):
+    rng = np.random.default_rng(1)
+    net = MLP(40, 10, "cuda", width=64, depth=2, seed=3)
+    for ly in net.layers:  # nonzero biases and a last layer of full scale
+        ly.b.copy_(torch.from_numpy((0.1 * rng.standard_normal(ly.b.shape[0])).astype(np.float32)))
+    net.layers[-1].w.mul_(10.0)
+    net._sync_bf16()
+    M, rows = 128, 100
+    X = np.zeros((M, net.dims[0]), np.float32)
+    X[:rows, :40] = rng.standard_normal((rows, 40))
+    T = rng.standard_normal((rows, 10)).astype(np.float32)
+    w = np.linspace(0.5, 2.0, 10).astype(np.float32)
+    Y = net.forward(dev(bf16_bits(X), torch.int16), keep=True)
+    dY = net.loss_grad(Y, dev(T, torch.float32), dev(w, torch.float32), rows)
+    net.backward(dY)
+    y_ref, gw_ref, gb_ref = reference(net, X, T.astype(np.float64), w.astype(np.float64), rows)
+    y = Y.cpu().numpy()
+    np.testing.assert_allclose(y[:, :10], y_ref[:, :10], rtol=2e-2, atol=2e-2 * np.abs(y_ref).max())
+    for l, ly in enumerate(net.layers):
+        g, gr = ly.gw.cpu().numpy(), gw_ref[l]
+        err = np.abs(g - gr).max() / np.abs(gr).max()
+        print(f"layer {l}: dW max rel to max {err:.2e}; db {np.abs(ly.gb.cpu().numpy() - gb_ref[l]).max():.2e}")
+        assert err < 3e-2
+        np.testing.assert_allclose(ly.gb.cpu().numpy(), gb_ref[l], rtol=3e-2, atol=3e-2 * np.abs(gb_ref[l]).max())
+    # the padding stays out: no gradient reaches padded inputs or outputs
+    assert float(net.layers[0].gw[:, 40:].abs().max()) == 0.0
+    assert float(net.layers[-1].gw[10:].abs().max()) == 0.0
+
+
+def test_adam_fits_a_smooth_map(gpu_ok):
+    rng = np.random.default_rng(2)
+    net = MLP(16, 4, "cuda", width=128, depth=3, seed=5)
+    P = rng.standard_normal((16, 4)) / 4.0
+    M = 1024
+    X = np.zeros((M, net.dims[0]), np.float32)
+    X[:, :16] = rng.uniform(-1.0, 1.0, (M, 16))
+    T = np.sin(X[:, :16] @ P).astype(np.float32)
+    Xd, Td, wd = dev(bf16_bits(X), torch.int16), dev(T, torch.float32), torch.ones(4, device="cuda")
+    losses = []
+    for it in range(300):
+        net.loss.zero_()
+        Y = net.forward(Xd, keep=True)
+        net.backward(net.loss_grad(Y, Td, wd, M))
+        net.adam(3e-3)
+        losses.append(float(net.loss) / (M * 4))
+    print(f"loss {losses[0]:.3e} -> {losses[-1]:.3e} (target variance {T.var():.3e})")
+    assert losses[-1] < 0.05 * losses[0]
+
+
+def c1_sequence(net, cells, seed, calls):
+    from maxichem_amd import workloads
+
+    d = workloads.config1(net.species, len(net.photo), cells=cells, seed=seed)
+    dark = np.zeros_like(d["J"])
+    return {"x": d["x"], "T": d["T"], "P": d["P"], "h": d["h"],
+            "calls": [(d["dt_relax"], dark)] + [(d["dt"], d["J"])] * calls}  # fmt: skip
+
+
+def test_learned_solver_keeps_the_plain_results_and_reports_its_gain(gpu_ok):
+    from maxichem_amd import standin
+    from maxichem_amd.kincuda import CudaSolver
+    from maxichem_amd.learned import LearnedSolver, Surrogate, generate
+
+    net = standin.load("NCHO")
+    with CudaSolver(net, "cuda", photo=True) as plain:
+        data = generate(plain, [c1_sequence(net, 4096, 11, 6), c1_sequence(net, 4096, 12, 6)])
+        sur = Surrogate(net, "cuda", seed=1)
+        sur.standardise(data)
+        before = sur.evaluate(data)
+        losses = sur.fit(data, epochs=6.0, batch=4096, lr=2e-3)
+        after = sur.evaluate(data)
+        print(f"surrogate on {data['x0'].shape[0]} calls: losses {[round(v, 4) for v in losses]}; mean abs error "
+              f"before {before} after {after}")
+        assert losses[-1] < losses[0] and after["dlog10x"] < before["dlog10x"]
+        # a held-out sequence: each call from the plain result, the plain and learned forms on the same inputs
+        sq = c1_sequence(net, 2048, 21, 4)
+        g = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+        forms = {"warm start and order": dict(propose=False), "with proposals": dict(propose=True)}
+        for name, kw in forms.items():
+            x, h, T, P = g(sq["x"]), g(sq["h"]), g(sq["T"]), g(sq["P"])
+            worst = 0.0
+            with LearnedSolver(net, "cuda", sur, photo=True, **kw) as learned:
+                tp = tl = 0.0
+                for k, (dt_np, J_np) in enumerate(sq["calls"]):
+                    dt, J = g(dt_np), g(J_np)
+                    want = plain(x, T, P, dt, h, J=J)
+                    tp += float(plain.tries.sum())
+                    got = learned(x, T, P, dt, h, J=J)
+                    tl += float(learned.tries.sum())
+                    big = want.x > 1e-20
+                    rel = float((got.x[big] / want.x[big] - 1.0).abs().max())
+                    worst = max(worst, rel)
+                    print(f"{name}, call {k}: max rel {rel:.2e}, done {int(got.done.sum())}/{int(want.done.sum())}, "
+                          f"tries plain {float(plain.tries.sum()):.0f} learned {float(learned.tries.sum()):.0f}, "
+                          f"recomputed {learned.fallback['recomputed']}")
+                    assert torch.equal(got.done, want.done)
+                    x, h = want.x, want.h
+            print(f"{name}: tries over the sequence plain {tp:.0f}, learned {tl:.0f} ({tp / max(tl, 1.0):.2f}x "
+                  f"fewer); worst species error against the plain form {worst:.2e}")
+            if not kw["propose"]:  # the warm start and the order change the steps, not the method
+                assert worst <= 1e-2

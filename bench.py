@@ -9,7 +9,10 @@ from the previous call's x and h, with per-branch J log-uniform 1e-9 to 1e-3 s^-
 and J = 0 elsewhere. All float64, inputs resident in HBM.
 
 A step is one call, solver(x, T, P, dt, h, J=J): the call kernel, the guard, the recovery and the count read.
-value = cells x K / wall time of the K calls. The roofline is the call kernel's: its algorithmic FP64 flops per launch
+--form both (the default) times the plain FP64 form and the learned form (a surrogate trained first on calls of the
+plain solver over other seeds) from the same start; value = cells x K / wall time of the K calls of the fastest form
+whose first timed call agrees with the plain form's to the parity bar (species above 1e-20 within relative 1e-2, the
+same done flags), and the line holds every form's figures. The roofline is the call kernel's: its algorithmic FP64 flops per launch
 (per-try work counted from the network's tables, times each cell's tries, which the kernel counts) over its average
 launch duration from HIP events on the launch stream. The CPU baseline times the FP64 oracle (OpenMP, one cell per
 thread) on a sample of the first timed call's cells and checks the GPU's result on them.
@@ -205,7 +208,19 @@ def main() -> int:
             f"average; mean tries per cell-call {np.mean([t.mean() for t in rr['tries']]):.2f}; last call done "
             f"{int(rr['last'].done.sum())}/{args.cells}")  # fmt: skip
 
-    headline = "learned" if "learned" in runs else "plain"
+    # the learned form against the plain one on the same start (its first timed call): the same error bar as the
+    # parity tests (species above 1e-20 within relative 1e-2, done flags equal)
+    agree = {}
+    if "plain" in runs and "learned" in runs:
+        a, b = runs["plain"]["first"], runs["learned"]["first"]
+        big = a.x > 1e-20
+        sh = lambda x: (x @ solver._tables["atoms"]) / (x @ solver._tables["atoms"]).sum(1, keepdim=True)  # noqa: E731
+        agree = {"max_rel_species_above_1e-20": float((b.x[big] / a.x[big] - 1.0).abs().max()),
+                 "max_share_diff": float((sh(a.x) - sh(b.x)).abs().max()),
+                 "done_match": bool(torch.equal(a.done, b.done)), "training_calls": ndata}  # fmt: skip
+    # the headline: the fastest form at the plain form's error
+    ok = {n for n in runs if n == "plain" or (agree and agree["done_match"] and agree["max_rel_species_above_1e-20"] <= 1e-2)}
+    headline = min(ok, key=lambda n: runs[n]["wall"])
     main_run = runs[headline]
     line = {
         "metric": "cells/s per call (NCHO, 64K cells): learned+FP32-LU vs plain FP64 at same error",
@@ -225,7 +240,7 @@ def main() -> int:
                         "dt = 300 s carrying x and h, J log-uniform 1e-9..1e-3 s^-1 on the lowest-P quarter",
             "network": f"{'stand-in ' if getattr(net, 'standin', True) else ''}{args.network}: "
                        f"{len(net.species)} species, {len(net.reac)} thermal, {nj} photolysis",
-            "cells": args.cells, "seed": 1, "form": headline, "rtol": 1e-3, "atol": 1e-25,
+            "cells": args.cells, "seed": 1, "form": headline, "forms_timed": list(runs), "rtol": 1e-3, "atol": 1e-25,
         },
         "forms": {},
         "roofline": {
@@ -245,15 +260,8 @@ def main() -> int:
             "frac_fp64_peak": rr["achieved"] / FP64_PEAK,
             "tries_per_cell_call": float(np.mean([t.mean() for t in rr["tries"]])),
         }  # fmt: skip
-    if "plain" in runs and "learned" in runs:
-        a, b = runs["plain"]["first"], runs["learned"]["first"]
-        big = a.x > 1e-20
-        rel = float((b.x[big] / a.x[big] - 1.0).abs().max())
-        sh = lambda x: (x @ solver._tables["atoms"]) / (x @ solver._tables["atoms"]).sum(1, keepdim=True)  # noqa: E731
-        line["forms"]["learned"]["vs_plain_first_call"] = {
-            "max_rel_species_above_1e-20": rel, "max_share_diff": float((sh(a.x) - sh(b.x)).abs().max()),
-            "done_match": bool(torch.equal(a.done, b.done)), "training_calls": ndata,
-        }  # fmt: skip
+    if agree:
+        line["forms"]["learned"]["vs_plain_first_call"] = agree
         line["forms"]["learned"]["speedup_vs_plain"] = runs["plain"]["wall"] / runs["learned"]["wall"]
     if not args.no_cpu:
         first = runs["plain"]["first"] if "plain" in runs else main_run["first"]

@@ -382,23 +382,35 @@ def generate(solver, sequences: list[dict], log=None) -> dict:
 class LearnedSolver(CudaSolver):
     """CudaSolver with the learned warm start of a trained Surrogate.
 
-    The surrogate's step size starts each cell (warm), its cost estimate orders the queue (costliest first), and its
-    proposed end state is tried first (propose): it is kept only when one error-controlled try of the whole dt from
-    it passes and moves it by no more than the tolerance. Every other cell is solved by the plain method from its
-    input, and the guard and the recovery judge every result as in the plain form. The call, results, counts and
-    refusals are CudaSolver's.
+    The surrogate's cost estimate orders the queue (costliest first). Its step size starts the cells that come
+    cold (warm="cold": a caller's h that is not finite or below COLD times dt, as a first call's is; warm="all":
+    every cell; warm="none"). Its proposed end state is tried first when propose is set: it is kept only when one
+    error-controlled try of the whole dt from it passes and moves it by no more than the tolerance. Every other cell
+    is solved by the plain method from its input, and the guard and the recovery judge every result as in the plain
+    form. The call, results, counts and refusals are CudaSolver's.
+
+    Proposals are off by default: a try from a state tests that the state is stationary over dt, not that it is the
+    cell's state at the end of dt, and where chemistry is slow a wrong proposal passes (measured on the NCHO
+    workload: species off by several times).
     """
 
-    def __init__(self, net, device, surrogate: Surrogate, *, drift=None, photo=False, propose: bool = True,
-                 warm: bool = True) -> None:  # fmt: skip
+    COLD = 1e-6
+
+    def __init__(self, net, device, surrogate: Surrogate, *, drift=None, photo=False, propose: bool = False,
+                 warm: str = "cold") -> None:  # fmt: skip
+        if warm not in ("cold", "all", "none"):
+            raise ValueError(f"warm: 'cold', 'all' or 'none' expected, got {warm!r}")
         super().__init__(net, device, drift=drift, photo=photo)
         self.surrogate, self.propose, self.warm = surrogate, propose, warm
         self.predicted_cost: torch.Tensor | None = None
 
     def _plan(self, x, T, P, dt, h_start, jx, B):
         h0, cost, xp = self.surrogate.predict(x, T, P, dt, jx, H_MIN)
-        if self.warm:
+        if self.warm == "all":
             h_start.copy_(h0)
+        elif self.warm == "cold":
+            cold = ~torch.isfinite(h_start) | (h_start < self.COLD * dt)
+            h_start.copy_(torch.where(cold, h0, h_start))
         self.predicted_cost = cost
         order = torch.argsort(cost, descending=True).to(torch.int32)
         return order, (xp if self.propose else None)

@@ -163,7 +163,8 @@ def test_learned_solver_keeps_the_plain_results_and_reports_its_gain(gpu_ok):
         # a held-out sequence: each call from the plain result, the plain and learned forms on the same inputs
         sq = c1_sequence(net, 2048, 21, 4)
         g = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
-        forms = {"warm start and order": dict(propose=False), "with proposals": dict(propose=True)}
+        forms = {"cold warm start and order": dict(propose=False), "warm start everywhere": dict(warm="all"),
+                 "with proposals": dict(propose=True)}  # fmt: skip
         for name, kw in forms.items():
             x, h, T, P = g(sq["x"]), g(sq["h"]), g(sq["T"]), g(sq["P"])
             worst = 0.0
@@ -185,5 +186,5 @@ def test_learned_solver_keeps_the_plain_results_and_reports_its_gain(gpu_ok):
                     x, h = want.x, want.h
             print(f"{name}: tries over the sequence plain {tp:.0f}, learned {tl:.0f} ({tp / max(tl, 1.0):.2f}x "
                   f"fewer); worst species error against the plain form {worst:.2e}")
-            if not kw["propose"]:  # the warm start and the order change the steps, not the method
+            if not kw.get("propose"):  # the warm start and the order change the steps, not the method
                 assert worst <= 1e-2

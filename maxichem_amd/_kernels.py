@@ -51,8 +51,12 @@ _FLAGS = ("FL_FALLOFF", "FL_SECOND_K0", "FL_THREE_BODY", "FL_TROE", "FL_EFF", "F
           "FL_NF_SHIFT", "FL_NR_SHIFT")  # fmt: skip
 
 
-def header(tab: Tables, cpw: int, drift: float) -> str:
-    """The network header: every size, offset and setting the kernel files read."""
+DEFAULT_BATCH = 8
+
+
+def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH) -> str:
+    """The network header: every size, offset and setting the kernel files read. cpw: cells (wavefronts) per
+    workgroup of the call kernel; batch: the rounds of a lane stream whose words are loaded together."""
     lay = tab.layout()
     nch = len(tab.chunk_words)
     rounds = [w.shape[0] for w in tab.chunk_words]
@@ -60,7 +64,7 @@ def header(tab: Tables, cpw: int, drift: float) -> str:
     ints = {
         "WAVE": C.WAVE, "NS": tab.ns, "NR": tab.nr, "NRT": tab.nrt, "NJ": tab.nj, "K0": tab.k0, "NT": tab.nt,
         "NEL": tab.nel, "RCW": C.RATE_ROW, "RROUNDS": -(-tab.nr // C.WAVE), "NDV": tab.ndv, "NPART": tab.npart,
-        "CPW": cpw, "GUARD_THREADS": GUARD_THREADS, "CHUNK": CHUNK, "NCH": nch,
+        "CPW": cpw, "STREAM_BATCH": batch, "GUARD_THREADS": GUARD_THREADS, "CHUNK": CHUNK, "NCH": nch,
         "SV_ROUNDS": tab.sv_words.shape[0], "DIAG_ROUNDS": tab.diag_words.shape[0],
         "LEAD_ROUNDS": tab.lead_words.shape[0], "REC_RETRIES": C.REC_RETRIES,
         "T_NEG": T_NEG, "T_DBL": T_DBL, "T_END": T_END, "T_DEST": T_DEST, "SV_NEG": SV_NEG, "SV_DBL": SV_DBL,
@@ -97,9 +101,9 @@ def mlp_source() -> str:
     return (KERNEL_DIR / MLP_FILE).read_text()
 
 
-def source(tab: Tables, cpw: int, drift: float) -> str:
+def source(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH) -> str:
     """The whole translation unit hiprtc compiles: the header, then the kernel files in order."""
-    parts = [header(tab, cpw, drift)]
+    parts = [header(tab, cpw, drift, batch)]
     for name in KERNEL_FILES:
         parts.append(f"// ---- {name} ----\n" + (KERNEL_DIR / name).read_text())
     return "\n".join(parts)

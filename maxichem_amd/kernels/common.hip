@@ -22,7 +22,6 @@ constexpr double TROE_D = 0.14;
 constexpr int REFUSED_COLD = 1;
 constexpr int REFUSED_BAD_INPUT = 2;
 constexpr int REFUSED_HOT = 3;
-constexpr int STREAM_BATCH = 8;  // rounds of a lane stream whose words are loaded together
 
 // Orders the LDS accesses of the wavefront's lanes: the wave executes in lockstep and LDS serves one wave's accesses
 // in issue order, so only the compiler must not move loads and stores across this point.

@@ -30,7 +30,8 @@ from ._constants import (
 from ._solver import Solver, check_inputs, check_photolysis
 
 __all__ = [
-    "ATOL", "COUNTS", "DRIFT", "REC_STEPS", "T_MAX", "T_MIN", "X_SUM_TOL", "CallResult", "CudaSolver", "LaunchResult",
+    "ATOL", "COUNTS", "DRIFT", "REC_STEPS", "T_MAX", "T_MIN", "X_SUM_TOL", "CallGraph", "CallResult", "CudaSolver",
+    "LaunchResult",
 ]  # fmt: skip
 
 
@@ -64,18 +65,30 @@ class CudaSolver(Solver):
         The guard's element-drift allowance (DRIFT if None). Keyword only, as photo.
     photo : bool
         The network's photolysis branches enter the call, which then takes J [B, branches].
+    variant : (int, int), optional
+        The call kernel's cells (wavefronts) per workgroup and stream batch; by default the autotuner's stored
+        choice for this network and device, else (1, 8).
+    tune : bool
+        Run the autotuner now (maxichem_amd.tune) and store its choice.
     """
 
-    def __init__(self, net, device, *, drift: float | None = None, photo: bool = False) -> None:
+    def __init__(self, net, device, *, drift: float | None = None, photo: bool = False,
+                 variant: tuple[int, int] | None = None, tune: bool = False) -> None:  # fmt: skip
         super().__init__(net, device, drift, photo)
         with torch.cuda.device(self.dev):
-            self._setup(net, photo)
+            if variant is None:
+                from . import tune as _tune
 
-    def _setup(self, net, photo: bool) -> None:
+                variant = _tune.choose(net, self.dev, photo, self.drift, run=tune)
+            self._setup(net, photo, variant)
+
+    def _setup(self, net, photo: bool, variant: tuple[int, int]) -> None:
         tab = _tables.build(net, photo)
         self.tab = tab
         self.ns, self.nr, self.nj = tab.ns, tab.nr, tab.nj
-        self.cpw = 1  # cells (wavefronts) per workgroup
+        self.cpw, self.batch = (int(v) for v in variant)  # cells (wavefronts) per workgroup, stream batch
+        if self.cpw not in (1, 2, 4, 8) or self.batch not in (2, 4, 8, 16):
+            raise ValueError(f"variant: cells per workgroup 1, 2, 4 or 8 and a batch of 2-16 expected, got {variant}")
         dev = self.dev.index
         info = _hip.device_info(dev)
         lay = tab.layout()
@@ -87,7 +100,7 @@ class CudaSolver(Solver):
                     f"one cell's {what} kernel needs {need} B of LDS per workgroup; the device has "
                     f"{info['lds_per_block']}"
                 )
-        code = _hip.compile_source(_kernels.source(tab, self.cpw, self.drift))
+        code = _hip.compile_source(_kernels.source(tab, self.cpw, self.drift, self.batch))
         self._module = _hip.Module(code, dev)
         self._call = self._module.function("mxc_call")
         self._guard = self._module.function("mxc_guard")
@@ -123,6 +136,7 @@ class CudaSolver(Solver):
             "private_bytes": _hip.attribute(self._call, _hip.ATTR_PRIVATE_BYTES),
             "cells_per_cu": self.max_blocks // cus * self.cpw, "max_blocks": self.max_blocks,
             "rec_blocks": self.rec_blocks, "compute_units": cus, "drift": self.drift, "tables": tab.digest(),
+            "variant": (self.cpw, self.batch),
         }  # fmt: skip
 
     def launch(
@@ -197,6 +211,11 @@ class CudaSolver(Solver):
                     ctypes.c_int(REC_STEPS * max_steps)]  # fmt: skip
         _hip.launch(self._recover, max(1, min(B, self.rec_blocks)), WAVE, self.lds_rec, stream, rec_args)
 
+    def capture(self, x, T, P, dt, h, *, rtol: float = DEFAULT_RTOL, atol: float = ATOL,
+                max_steps: int = DEFAULT_MAX_STEPS, J=None) -> "CallGraph":  # fmt: skip
+        """This call on these tensors captured in a HIP graph (CallGraph): replay() repeats it on their contents."""
+        return CallGraph(self, x, T, P, dt, h, rtol=rtol, atol=atol, max_steps=max_steps, J=J)
+
     def __call__(
         self,
         x: torch.Tensor,
@@ -213,3 +232,73 @@ class CudaSolver(Solver):
         """Advance each cell by dt: launch without the flag in its result."""
         r = self.launch(x, T, P, dt, h, rtol=rtol, atol=atol, max_steps=max_steps, J=J)
         return CallResult(r.x, r.h, r.done, r.steps)
+
+
+class CallGraph:
+    """One call of a solver captured in a HIP graph on fixed tensors (a GCM's time loop calls with the same tensors
+    every step): replay() runs the call kernel, guard and recovery again on what the input tensors hold then, with
+    one launch, and returns the graph's own output tensors, which the next replay overwrites. The counts, the
+    refusals and check() behave as in an eager call; the queue is ordered by the previous replay's tries, set on the
+    host side before each replay. Building it runs the call once, off the graph, without adding to the counts."""
+
+    def __init__(self, solver: CudaSolver, x, T, P, dt, h, *, rtol: float = DEFAULT_RTOL, atol: float = ATOL,
+                 max_steps: int = DEFAULT_MAX_STEPS, J=None) -> None:  # fmt: skip
+        solver._require_open()
+        self.solver = solver
+        dev = solver.dev
+        self.B = B = check_inputs(dev, solver.ns, x, T, P, dt, h, rtol, atol, max_steps)
+        for name, a in (("x", x), ("T", T), ("P", P), ("dt", dt), ("h", h)):
+            if not a.is_contiguous():
+                raise ValueError(f"{name}: a contiguous tensor expected for a captured call")
+        self.jx = check_photolysis(J, B, solver.nj, dev)
+        if J is not None and self.jx is not None and self.jx.data_ptr() != J.data_ptr():
+            raise ValueError("J: a contiguous tensor expected for a captured call")
+        self.inputs = (x, T, P, dt, h)
+        self.settings = (rtol, atol, max_steps)
+        with torch.cuda.device(dev):
+            self.xo, self.ho = torch.empty_like(x), torch.empty_like(h)
+            self.done, self.redo, self.lst, self.status = (torch.empty(B, dtype=torch.int32, device=dev)
+                                                           for _ in range(4))  # fmt: skip
+            self.nok = torch.empty(B, dtype=torch.float64, device=dev)
+            self.tries = torch.zeros(B, dtype=torch.float64, device=dev)
+            self.order = torch.arange(B, dtype=torch.int32, device=dev)
+            self._ordered = False
+            self.graph = torch.cuda.CUDAGraph()
+            side = torch.cuda.Stream(dev)
+            side.wait_stream(torch.cuda.current_stream(dev))
+            with torch.cuda.stream(side):
+                self._device_part(count=False)  # once eagerly, off the graph: the kernels' first launch
+            torch.cuda.current_stream(dev).wait_stream(side)
+            torch.cuda.synchronize(dev)
+            with torch.cuda.graph(self.graph):
+                self._device_part()
+
+    def _device_part(self, count: bool = True) -> None:
+        s = self.solver
+        x, T, P, dt, h = self.inputs
+        rtol, atol, max_steps = self.settings
+        self.ho.copy_(h)
+        self.tries.zero_()  # fill kernels: no memset node in the graph
+        s.count.zero_()
+        saved, s.tries = s.tries, self.tries
+        stream = torch.cuda.current_stream(s.dev).cuda_stream
+        s._launch_all(x, T, P, dt, h, self.xo, self.ho, self.done, self.nok, self.status, self.redo, self.lst,
+                      self.jx, self.B, rtol, atol, max_steps, stream, self.order)  # fmt: skip
+        s.tries = saved
+        if count:
+            s.total += s.count
+
+    def replay(self) -> LaunchResult:
+        s = self.solver
+        s._require_open()
+        s.flag = None
+        with torch.cuda.device(s.dev):
+            if self._ordered:  # the costliest cells of the previous replay first
+                self.order.copy_(torch.argsort(self.tries, descending=True).to(torch.int32))
+            self.graph.replay()
+            self._ordered = True
+            x, T, P, dt, _ = self.inputs
+            s.tries = self.tries
+            s._refuse(x, T, P, dt, self.jx, self.status)
+        s.flag = self.redo
+        return LaunchResult(self.xo, self.ho, self.done.bool(), self.nok, self.redo)

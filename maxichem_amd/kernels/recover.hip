@@ -47,7 +47,19 @@ __device__ void dense_lu(double* W, int* pr, double* dinv, int lane) {
     for (int i = k + 1 + lane; i < NS; i += WAVE) {
       const double l = W[i * NS + k] * dk;
       W[i * NS + k] = l;
-      for (int j = k + 1; j < NS; j++) W[i * NS + j] = fma(-l, W[k * NS + j], W[i * NS + j]);
+      // eight columns at a time: their loads go out together instead of one LDS round trip per column
+      int j = k + 1;
+      for (; j + 8 <= NS; j += 8) {
+        double u[8], w[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          u[q] = W[k * NS + j + q];
+          w[q] = W[i * NS + j + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) W[i * NS + j + q] = fma(-l, u[q], w[q]);
+      }
+      for (; j < NS; j++) W[i * NS + j] = fma(-l, W[k * NS + j], W[i * NS + j]);
     }
     wave_sync();
   }
@@ -58,7 +70,8 @@ __device__ void dense_solve(const double* W, const int* pr, const double* dinv, 
   double a[2];
 #pragma unroll
   for (int p = 0; p < 2; p++) a[p] = lane + WAVE * p < NS ? y[pr[lane + WAVE * p]] : 0.0;
-  for (int k = 0; k < NS - 1; k++) {
+#pragma unroll 4
+  for (int k = 0; k < NS - 1; k++) {  // unrolled a little: the factor loads of later steps go out early
     const double yk = readlane(k < WAVE ? a[0] : a[1], k & (WAVE - 1));
 #pragma unroll
     for (int p = 0; p < 2; p++) {
@@ -66,6 +79,7 @@ __device__ void dense_solve(const double* W, const int* pr, const double* dinv, 
       if (i > k && i < NS) a[p] = fma(-W[i * NS + k], yk, a[p]);
     }
   }
+#pragma unroll 4
   for (int k = NS - 1; k > 0; k--) {
     const double zk = readlane(k < WAVE ? a[0] : a[1], k & (WAVE - 1)) * dinv[k];
 #pragma unroll

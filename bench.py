@@ -9,10 +9,11 @@ from the previous call's x and h, with per-branch J log-uniform 1e-9 to 1e-3 s^-
 and J = 0 elsewhere. All float64, inputs resident in HBM.
 
 A step is one call, solver(x, T, P, dt, h, J=J): the call kernel, the guard, the recovery and the count read.
---form both (the default) times the plain FP64 form and the learned form (a surrogate trained first on calls of the
-plain solver over other seeds) from the same start; value = cells x K / wall time of the K calls of the fastest form
-whose first timed call agrees with the plain form's to the parity bar (species above 1e-20 within relative 1e-2, the
-same done flags), and the line holds every form's figures. The roofline is the call kernel's: its algorithmic FP64 flops per launch
+--form all (the default) times, from the same start, the plain FP64 form, the mixed-precision form (FP32 trailing LU,
+FP64 refinement), the learned form (a surrogate trained first on calls of the plain solver over other seeds) and the
+learned form with FP32 LU; value = cells x K / wall time of the K calls of the fastest form whose first timed call
+agrees with the plain form's to the parity bar (species above 1e-20 within relative 1e-2, the same done flags), and
+the line holds every form's figures. The roofline is the call kernel's: its algorithmic FP64 flops per launch
 (per-try work counted from the network's tables, times each cell's tries, which the kernel counts) over its average
 launch duration from HIP events on the launch stream. The CPU baseline times the FP64 oracle (OpenMP, one cell per
 thread) on a sample of the first timed call's cells and checks the GPU's result on them.
@@ -40,8 +41,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--network", default="NCHO", choices=("NCHO", "SNCHO"))
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="leave out the CPU baseline (profiling runs)")
-    ap.add_argument("--form", default="both", choices=("plain", "learned", "both"),
-                    help="plain FP64, the learned warm start, or both on the same start (value: the learned form)")
+    ap.add_argument("--form", default="all", choices=("plain", "mixed", "learned", "learned_mixed", "all"),
+                    help="plain FP64; mixed (FP32 trailing LU, FP64 refinement); learned (the learned warm start, FP64);"
+                         " learned_mixed (learned + FP32 LU, the metric's form); all: each from the same start")
     ap.add_argument("--train-cells", type=int, default=16384, help="cells per training sequence")
     ap.add_argument("--train-sequences", type=int, default=6, help="training sequences (seeds 101..)")
     ap.add_argument("--train-epochs", type=float, default=4.0)
@@ -254,42 +256,57 @@ def main() -> int:
         return flops.call_flops(solver.tab, t)
 
     runs = {}
-    if args.form in ("plain", "both"):
+    forms = ("plain", "mixed", "learned", "learned_mixed") if args.form == "all" else (args.form,)
+    if "plain" in forms:
         runs["plain"] = timed(solver, x, T, P, dt, h, J, args.steps, flops_of)
-    if args.form in ("learned", "both"):
+    slow_steps = min(args.steps, 2)  # the mixed forms take hundreds of times the plain form's tries per call
+    if "mixed" in forms:
+        mixed = CudaSolver(net, dev, photo=True, precision="mixed")
+        runs["mixed"] = timed(mixed, x, T, P, dt, h, J, slow_steps, flops_of)
+        mixed.close()
+    ndata = 0
+    if "learned" in forms or "learned_mixed" in forms:
         from maxichem_amd.learned import LearnedSolver
 
         sur, ndata = train_surrogate(net, solver, args, log)
-        learned = LearnedSolver(net, dev, sur, photo=True)
-        learned(x, T, P, dt, h, J=J)  # warm-up of the learned path (its first call from the same start)
-        runs["learned"] = timed(learned, x, T, P, dt, h, J, args.steps, flops_of)
+        for name, precision in (("learned", "fp64"), ("learned_mixed", "mixed")):
+            if name in forms:
+                ls = LearnedSolver(net, dev, sur, photo=True, precision=precision)
+                if precision == "fp64":
+                    ls(x, T, P, dt, h, J=J)  # warm-up of the learned path (its first call from the same start)
+                runs[name] = timed(ls, x, T, P, dt, h, J, args.steps if precision == "fp64" else slow_steps, flops_of)
+                ls.close()
     for name, rr in runs.items():
-        log(f"{name}: {args.steps} timed calls {rr['wall']:.3f} s; call kernel {rr['kernel_ms'].mean():.3f} ms "
+        log(f"{name}: {len(rr['tries'])} timed calls {rr['wall']:.3f} s; call kernel {rr['kernel_ms'].mean():.3f} ms "
             f"average; mean tries per cell-call {np.mean([t.mean() for t in rr['tries']]):.2f}; last call done "
             f"{int(rr['last'].done.sum())}/{args.cells}")  # fmt: skip
 
-    # the learned form against the plain one on the same start (its first timed call): the same error bar as the
-    # parity tests (species above 1e-20 within relative 1e-2, done flags equal)
+    # every form against the plain one on the same start (its first timed call): the parity bar (species above
+    # 1e-20 within relative 1e-2, done flags equal)
     agree = {}
-    if "plain" in runs and "learned" in runs:
-        a, b = runs["plain"]["first"], runs["learned"]["first"]
+    if "plain" in runs:
+        a = runs["plain"]["first"]
         big = a.x > 1e-20
         sh = lambda x: (x @ solver._tables["atoms"]) / (x @ solver._tables["atoms"]).sum(1, keepdim=True)  # noqa: E731
-        agree = {"max_rel_species_above_1e-20": float((b.x[big] / a.x[big] - 1.0).abs().max()),
-                 "max_share_diff": float((sh(a.x) - sh(b.x)).abs().max()),
-                 "done_match": bool(torch.equal(a.done, b.done)), "training_calls": ndata}  # fmt: skip
+        for name, rr in runs.items():
+            if name != "plain":
+                b = rr["first"]
+                agree[name] = {"max_rel_species_above_1e-20": float((b.x[big] / a.x[big] - 1.0).abs().max()),
+                               "max_share_diff": float((sh(a.x) - sh(b.x)).abs().max()),
+                               "done_match": bool(torch.equal(a.done, b.done))}  # fmt: skip
     # the headline: the fastest form at the plain form's error
-    ok = {n for n in runs if n == "plain" or (agree and agree["done_match"] and agree["max_rel_species_above_1e-20"] <= 1e-2)}
-    headline = min(ok, key=lambda n: runs[n]["wall"])
+    ok = {n for n in runs if n == "plain" or (n in agree and agree[n]["done_match"]
+                                              and agree[n]["max_rel_species_above_1e-20"] <= 1e-2)}  # fmt: skip
+    headline = min(ok, key=lambda n: runs[n]["wall"] / len(runs[n]["tries"]))
     main_run = runs[headline]
     line = {
         "metric": "cells/s per call (NCHO, 64K cells): learned+FP32-LU vs plain FP64 at same error",
-        "value": args.cells * args.steps / main_run["wall"],
+        "value": args.cells * len(main_run["tries"]) / main_run["wall"],
         "unit": "cells/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": len(main_run["tries"]),
         "warmup": args.warmup,
-        "ms_per_step": main_run["wall"] / args.steps * 1e3,
+        "ms_per_step": main_run["wall"] / len(main_run["tries"]) * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -314,15 +331,19 @@ def main() -> int:
         "cpu_baseline": None,
     }
     for name, rr in runs.items():
+        k = len(rr["tries"])
         line["forms"][name] = {
-            "cells_per_s": args.cells * args.steps / rr["wall"], "ms_per_call": rr["wall"] / args.steps * 1e3,
+            "cells_per_s": args.cells * k / rr["wall"], "ms_per_call": rr["wall"] / k * 1e3, "calls_timed": k,
             "kernel_ms": float(rr["kernel_ms"].mean()), "fp64_tflops": rr["achieved"],
             "frac_fp64_peak": rr["achieved"] / FP64_PEAK,
             "tries_per_cell_call": float(np.mean([t.mean() for t in rr["tries"]])),
         }  # fmt: skip
-    if agree:
-        line["forms"]["learned"]["vs_plain_first_call"] = agree
-        line["forms"]["learned"]["speedup_vs_plain"] = runs["plain"]["wall"] / runs["learned"]["wall"]
+    for name, a in agree.items():
+        line["forms"][name]["vs_plain_first_call"] = a
+        line["forms"][name]["speedup_vs_plain"] = (runs["plain"]["wall"] / len(runs["plain"]["tries"])) / (
+            runs[name]["wall"] / len(runs[name]["tries"]))
+    if ndata:
+        line["forms"]["training_calls"] = ndata
     if not args.no_cpu:
         first = runs["plain"]["first"] if "plain" in runs else main_run["first"]
         first_np = {"x": first.x.cpu().numpy(), "done": first.done.cpu().numpy()}

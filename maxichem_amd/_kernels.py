@@ -54,10 +54,11 @@ _FLAGS = ("FL_FALLOFF", "FL_SECOND_K0", "FL_THREE_BODY", "FL_TROE", "FL_EFF", "F
 DEFAULT_BATCH = 8
 
 
-def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH) -> str:
+def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixed: bool = False) -> str:
     """The network header: every size, offset and setting the kernel files read. cpw: cells (wavefronts) per
-    workgroup of the call kernel; batch: the rounds of a lane stream whose words are loaded together."""
-    lay = tab.layout()
+    workgroup of the call kernel; batch: the rounds of a lane stream whose words are loaded together; mixed: the
+    mixed-precision form (FP32 trailing factors, FP64 refinement)."""
+    lay = tab.layout(mixed)
     nch = len(tab.chunk_words)
     rounds = [w.shape[0] for w in tab.chunk_words]
     offsets = [sum(rounds[:c]) for c in range(nch)]
@@ -87,6 +88,7 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH) -> s
         f"__device__ constexpr int CH_ROUNDS[{nch}] = {{{', '.join(map(str, rounds))}}};",
         f"__device__ constexpr int CH_OFF[{nch}] = {{{', '.join(map(str, offsets))}}};",
         f"#define HAS_TROE {int(tab.has_troe)}",
+        f"#define MIXED {int(mixed)}",
         f"#define HAS_EFF {int(tab.has_eff)}",
         "#define DNAN __longlong_as_double(0x7ff8000000000000ll)",
     ]
@@ -101,9 +103,10 @@ def mlp_source() -> str:
     return (KERNEL_DIR / MLP_FILE).read_text()
 
 
-def source(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH) -> str:
-    """The whole translation unit hiprtc compiles: the header, then the kernel files in order."""
-    parts = [header(tab, cpw, drift, batch)]
-    for name in KERNEL_FILES:
+def source(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixed: bool = False) -> str:
+    """The whole translation unit hiprtc compiles: the header, then the kernel files in order (mixed.hip after
+    common.hip in the mixed-precision form)."""
+    parts = [header(tab, cpw, drift, batch, mixed)]
+    for name in KERNEL_FILES[:1] + (("mixed.hip",) if mixed else ()) + KERNEL_FILES[1:]:
         parts.append(f"// ---- {name} ----\n" + (KERNEL_DIR / name).read_text())
     return "\n".join(parts)

@@ -165,11 +165,12 @@ class Tables:
         return h.hexdigest()[:16]
 
     # ---- the kernel's LDS layout (doubles per cell) ----
-    def layout(self) -> dict[str, int]:
+    def layout(self, mixed: bool = False) -> dict[str, int]:
         """Offsets (doubles) in a cell's LDS region: LEAD [K0][NS], A21 [NT][K0], n [NS + 1], the rate
         coefficients KF, KR [NR] (kept for the call), dv [NDV + 1], then a union of what the rhs uses (v [NR + 1],
         pieces [NPART]) with what the Jacobian and the factorization use after it (the trailing diagonal [NT], staging
-        [NT][CHUNK]); G / RT [NS + 1] sits at its start while the rates are computed."""
+        [NT][CHUNK]); G / RT [NS + 1] sits at its start while the rates are computed. mixed: the mixed-precision
+        form's exchange vectors [2][NS + 1] follow."""
         a21 = self.k0 * self.ns
         nvec = a21 + self.nt * self.k0
         kf = nvec + self.ns + 1
@@ -181,9 +182,12 @@ class Tables:
         diag = union
         stage = diag + self.nt
         size = max(part + self.npart, stage + self.nt * CHUNK, union + self.ns + 1)
+        xv = size  # the mixed-precision form's exchange vectors [2][NS + 1]
+        if mixed:
+            size += 2 * (self.ns + 1)
         size += size & 1  # keep the next cell's region 16-byte aligned
         return {"A21": a21, "NVEC": nvec, "KF": kf, "KR": kr, "DV": dv, "UNION": union, "V": v, "PART": part,
-                "DIAG": diag, "STAGE": stage, "SIZE": size}
+                "DIAG": diag, "STAGE": stage, "XV": xv, "SIZE": size}
 
 
 def build(net: VulcanNetwork, photo: bool = False) -> Tables:

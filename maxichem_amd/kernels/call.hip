@@ -17,7 +17,11 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
   double* ws = lds + wv * L_SIZE;
   const int waves = (int)gridDim.x * CPW;
   int q = (int)blockIdx.x * CPW + wv;
+#if MIXED
+  float c[NT];  // the trailing block of M0 and its factors, FP32 in the mixed-precision form
+#else
   double c[NT];
+#endif
   while (q < B) {
     const int cell = order ? order[q] : q;
     const double Tc = Ta[cell], dt = dta[cell], nd = Pa[cell] / (KBOLTZ * Tc);
@@ -58,10 +62,22 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
       const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
       jacobian(c, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
       add_diagonal(c, igh, ws, lane);
-      double dl, dtr;
+      double dl;
+#if MIXED
+      float dtr;
+      factor_lead(c, dl, ws, lane);
+      factor_trailing32(c, dtr, ws, lane);
+      const double wl = atol + rtol * fabs(Nl), wt = atol + rtol * fabs(Nt);  // the refinement's error weights
+      bool lin_ok = true;  // every solve's refinement converged; else the try is rejected as a non-finite one
+#define MXC_SOLVE(NV, YL, YT) lin_ok = msolve<NV>(YL, YT, c, dl, dtr, igh, wl, wt, net, ws, lane) && lin_ok
+#else
+      double dtr;
       factor(c, dl, dtr, ws, lane);
+      constexpr bool lin_ok = true;
+#define MXC_SOLVE(NV, YL, YT) solve<NV>(YL, YT, c, dl, dtr, ws, lane)
+#endif
       double y2l[2] = {Fl, ul}, y2t[2] = {Ft, ut};
-      solve<2>(y2l, y2t, c, dl, dtr, ws, lane);
+      MXC_SOLVE(2, y2l, y2t);
       const double Ul = y2l[1], Ut = y2t[1];  // M0^-1 w
       const double c0 = 1.0 / (1.0 - vsum(Ul, Ut, lane));
       double K1l = y2l[0], K1t = y2t[0];
@@ -69,14 +85,14 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
       K1l += sk * Ul;
       K1t += sk * Ut;
       double y1l[1] = {fma(RODAS_C21 * ih, K1l, Fl)}, y1t[1] = {fma(RODAS_C21 * ih, K1t, Ft)};
-      solve<1>(y1l, y1t, c, dl, dtr, ws, lane);
+      MXC_SOLVE(1, y1l, y1t);
       sk = vsum(y1l[0], y1t[0], lane) * c0;
       const double K2l = y1l[0] + sk * Ul, K2t = y1t[0] + sk * Ut;
       double Yl = Nl + 2.0 * K1l, Yt = Nt + 2.0 * K1t;
       rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
       y1l[0] = fma(ih, K1l - K2l, Fl);
       y1t[0] = fma(ih, K1t - K2t, Ft);
-      solve<1>(y1l, y1t, c, dl, dtr, ws, lane);
+      MXC_SOLVE(1, y1l, y1t);
       sk = vsum(y1l[0], y1t[0], lane) * c0;
       const double K3l = y1l[0] + sk * Ul, K3t = y1t[0] + sk * Ut;
       Yl = Nl + 2.0 * K1l + K3l;
@@ -84,11 +100,16 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
       rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
       y1l[0] = fma(ih, fma(-RODAS_C43, K3l, K1l - K2l), Fl);
       y1t[0] = fma(ih, fma(-RODAS_C43, K3t, K1t - K2t), Ft);
-      solve<1>(y1l, y1t, c, dl, dtr, ws, lane);
+      MXC_SOLVE(1, y1l, y1t);
       sk = vsum(y1l[0], y1t[0], lane) * c0;
       const double K4l = y1l[0] + sk * Ul, K4t = y1t[0] + sk * Ut;
+#undef MXC_SOLVE
       bool ok = false;
-      const double en = try_error(Nl, Nt, K1l, K1t, K3l, K3t, K4l, K4t, Yl, Yt, s.t == 0.0, rtol, atol, lane, ok);
+      double en = try_error(Nl, Nt, K1l, K1t, K3l, K3t, K4l, K4t, Yl, Yt, s.t == 0.0, rtol, atol, lane, ok);
+      if (!lin_ok) {
+        ok = false;
+        en = DNAN;
+      }
       if (verifying) {
         // the proposal is kept when the try passes and moves it by no more than the tolerance: the end state is
         // then stationary over dt; otherwise the cell starts again from its input

@@ -325,40 +325,23 @@ __device__ __forceinline__ void jac_stream(const u32* __restrict__ words, int ro
 
 // -Jn into LEAD / A21 (LDS) and the trailing block c (registers), from the dv terms that rhs<true> left in DV; then
 // u = w = S v / nd - Jn N / n_tot, the constant-pressure part of the exact Jacobian (lead / trailing parts).
-__device__ __forceinline__ void jacobian(double (&c)[NT], double Nl, double Nt, double u0l, double u0t, double ntot,
+template <typename T>
+__device__ __forceinline__ void jacobian(T (&c)[NT], double Nl, double Nt, double u0l, double u0t, double ntot,
                                          double& ul, double& ut, const Net* net, double* ws, int lane) {
   double* DV = ws + L_DV;
   for (int s = lane; s < L_NVEC; s += WAVE) ws[s] = 0.0;  // LEAD and A21
   if (lane < NT) ws[L_DIAG + lane] = 0.0;                   // a species in no reaction has no diagonal term
   if (lane == 0) DV[NDV] = 0.0;                             // the padding words' term
+  // the state N in the exchange vector (n's place: free once the dv terms are in DV)
+  double* xv = ws + L_NVEC;
+  if (lane < K0) xv[lane] = Nl;
+  if (lane < NT) xv[K0 + lane] = Nt;
   wave_sync();
   jac_stream(opaque(net)->diagw, DIAG_ROUNDS, DV, ws, lane);
   jac_stream(opaque(net)->leadw, LEAD_ROUNDS, DV, ws, lane);
   wave_sync();
-  const double cd = lane < NT ? ws[L_DIAG + lane] : 0.0;
-  double* stage = ws + L_STAGE;
-#pragma unroll
-  for (int ch = 0; ch < NCH; ch++) {
-    if (lane < NT) {
-#pragma unroll
-      for (int j = 0; j < CHUNK; j++) stage[lane * CHUNK + j] = 0.0;
-      // the chunk streams hold no diagonal entry: a trailing row's own diagonal goes in by its lane
-      if (lane / CHUNK == ch) stage[lane * CHUNK + lane % CHUNK] = cd;
-    }
-    wave_sync();
-    jac_stream(opaque(net)->chw + CH_OFF[ch] * WAVE, CH_ROUNDS[ch], DV, ws, lane);
-    wave_sync();
-#pragma unroll
-    for (int j = 0; j < CHUNK; j++)
-      if (ch * CHUNK + j < NT) c[ch * CHUNK + j] = lane < NT ? stage[lane * CHUNK + j] : 0.0;
-  }
-  // -Jn N: the lead columns through the exchange vector (n's place, free until the next rhs), the trailing
-  // columns of the trailing rows with N_j taken from lane j
-  double* xv = ws + L_NVEC;
-  wave_sync();
-  if (lane < K0) xv[lane] = Nl;
-  if (lane < NT) xv[K0 + lane] = Nt;
-  wave_sync();
+  // -Jn N in FP64, from the entries before the trailing block takes them in T: the lead rows and the lead columns
+  // of the trailing rows through the exchange vector, then the trailing columns chunk by chunk, N_j from lane j
   double ml = 0.0, mt = 0.0;
   if (K0 > 0) {
 #pragma unroll 1
@@ -376,8 +359,27 @@ __device__ __forceinline__ void jacobian(double (&c)[NT], double Nl, double Nt, 
 #pragma unroll 1
     for (int k = 0; k < K0; k++) mt = fma(lane < NT ? ws[L_A21 + lane * K0 + k] : 0.0, xv[k], mt);
   }
+  const double cd = lane < NT ? ws[L_DIAG + lane] : 0.0;
+  double* stage = ws + L_STAGE;
 #pragma unroll
-  for (int j = 0; j < NT; j++) mt = fma(c[j], readlane(Nt, j), mt);
+  for (int ch = 0; ch < NCH; ch++) {
+    if (lane < NT) {
+#pragma unroll
+      for (int j = 0; j < CHUNK; j++) stage[lane * CHUNK + j] = 0.0;
+      // the chunk streams hold no diagonal entry: a trailing row's own diagonal goes in by its lane
+      if (lane / CHUNK == ch) stage[lane * CHUNK + lane % CHUNK] = cd;
+    }
+    wave_sync();
+    jac_stream(opaque(net)->chw + CH_OFF[ch] * WAVE, CH_ROUNDS[ch], DV, ws, lane);
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < CHUNK; j++)
+      if (ch * CHUNK + j < NT) {
+        const double v = lane < NT ? stage[lane * CHUNK + j] : 0.0;
+        c[ch * CHUNK + j] = (T)v;
+        mt = fma(v, readlane(Nt, ch * CHUNK + j), mt);
+      }
+  }
   ul = u0l + ml / ntot;
   ut = lane < NT ? u0t + mt / ntot : 0.0;
   wave_sync();
@@ -386,16 +388,18 @@ __device__ __forceinline__ void jacobian(double (&c)[NT], double Nl, double Nt, 
 // ---- the unpivoted LU of M0 in the elimination order, and its solves ----
 
 // Adds 1 / (gamma h) to M0's diagonal: the lead rows in LDS, the trailing block in registers.
-__device__ __forceinline__ void add_diagonal(double (&c)[NT], double igh, double* ws, int lane) {
+template <typename T>
+__device__ __forceinline__ void add_diagonal(T (&c)[NT], double igh, double* ws, int lane) {
   if (lane < K0) ws[lane * NS + lane] += igh;
 #pragma unroll
-  for (int j = 0; j < NT; j++) c[j] += j == lane ? igh : 0.0;
+  for (int j = 0; j < NT; j++) c[j] += j == lane ? (T)igh : (T)0;
   wave_sync();
 }
 
-// LU of the lead columns (right-looking, dense in LDS), the Schur update of the trailing block, then the trailing
-// block's LU in registers. dl: 1 / U_ii of lead row lane; dt: of trailing row lane.
-__device__ __forceinline__ void factor(double (&c)[NT], double& dl, double& dt, double* ws, int lane) {
+// LU of the lead columns (right-looking, dense in LDS) and the Schur update of the trailing block (in T). dl: 1 / U_ii
+// of lead row lane.
+template <typename T>
+__device__ __forceinline__ void factor_lead(T (&c)[NT], double& dl, double* ws, int lane) {
   double* A21 = ws + L_A21;
   dl = 1.0;
 #pragma unroll 1
@@ -421,9 +425,13 @@ __device__ __forceinline__ void factor(double (&c)[NT], double& dl, double& dt, 
       const double a = A21[lane * K0 + k];
       const double* uk = ws + k * NS + K0;
 #pragma unroll
-      for (int j = 0; j < NT; j++) c[j] = fma(-a, uk[j], c[j]);
+      for (int j = 0; j < NT; j++) c[j] = (T)fma(-a, uk[j], (double)c[j]);
     }
   }
+}
+
+// The trailing block's LU in registers (FP64). dt: 1 / U_ii of trailing row lane.
+__device__ __forceinline__ void factor_trailing(double (&c)[NT], double& dt, double* ws, int lane) {
   // the trailing block: at pivot k every lane takes row k's tail from lane k by v_readlane, as wave-uniform values
   // in scalar registers (no LDS round trip, no wait)
   // 1 / U_kk goes through LDS (the union region is free here): a select per step into a register would have the
@@ -445,10 +453,17 @@ __device__ __forceinline__ void factor(double (&c)[NT], double& dl, double& dt, 
   wave_sync();
 }
 
-// y <- M0^-1 y for NV vectors (lead / trailing parts), with factor's factors.
+// LU of the lead columns (right-looking, dense in LDS), the Schur update of the trailing block, then the trailing
+// block's LU in registers. dl: 1 / U_ii of lead row lane; dt: of trailing row lane.
+__device__ __forceinline__ void factor(double (&c)[NT], double& dl, double& dt, double* ws, int lane) {
+  factor_lead(c, dl, ws, lane);
+  factor_trailing(c, dt, ws, lane);
+}
+
+// y <- M0^-1 y for NV vectors (lead / trailing parts), with factor's factors, in three parts: the lead columns of L,
+// the trailing block, the lead rows of U.
 template <int NV>
-__device__ __forceinline__ void solve(double (&yl)[NV], double (&yt)[NV], const double (&c)[NT], double dl, double dt,
-                                      double* ws, int lane) {
+__device__ __forceinline__ void solve_lead_forward(double (&yl)[NV], double (&yt)[NV], double* ws, int lane) {
   const double* A21 = ws + L_A21;
 #pragma unroll 1
   for (int k = 0; k < K0; k++) {  // unit L, lead columns
@@ -461,6 +476,10 @@ __device__ __forceinline__ void solve(double (&yl)[NV], double (&yt)[NV], const 
       yt[v] = fma(-tk, yk, yt[v]);
     }
   }
+}
+
+template <int NV>
+__device__ __forceinline__ void solve_trailing(double (&yt)[NV], const double (&c)[NT], double dt, int lane) {
   // the trailing block's solves: at step k every lane takes lane k's final value by v_readlane and is updated with
   // its factor entry in column k, unmasked; lane k's value is recorded before its own update spoils it (and the
   // lanes already recorded are spoilt freely), so the chain from step to step is a readlane and an fma. pin keeps
@@ -491,6 +510,11 @@ __device__ __forceinline__ void solve(double (&yl)[NV], double (&yt)[NV], const 
   }
 #pragma unroll
   for (int v = 0; v < NV; v++) yt[v] *= dt;
+}
+
+template <int NV>
+__device__ __forceinline__ void solve_lead_backward(double (&yl)[NV], const double (&yt)[NV], double dl, double* ws,
+                                                    int lane) {
   if (K0 > 0) {
     // U of the lead rows, trailing columns: the trailing solution through LDS (the union region is free between
     // the right-hand sides), a batch of columns at a time
@@ -524,6 +548,14 @@ __device__ __forceinline__ void solve(double (&yl)[NV], double (&yt)[NV], const 
 #pragma unroll
     for (int v = 0; v < NV; v++) yl[v] *= dl;
   }
+}
+
+template <int NV>
+__device__ __forceinline__ void solve(double (&yl)[NV], double (&yt)[NV], const double (&c)[NT], double dl, double dt,
+                                      double* ws, int lane) {
+  solve_lead_forward(yl, yt, ws, lane);
+  solve_trailing(yt, c, dt, lane);
+  solve_lead_backward(yl, yt, dl, ws, lane);
 }
 
 __device__ __forceinline__ double vsum(double yl, double yt, int lane) {

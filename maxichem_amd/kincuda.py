@@ -70,11 +70,18 @@ class CudaSolver(Solver):
         choice for this network and device, else (1, 8).
     tune : bool
         Run the autotuner now (maxichem_amd.tune) and store its choice.
+    precision : str
+        "fp64" (the method as the reference computes it) or "mixed": the trailing block of M0 factored and solved in
+        FP32, each solve refined with FP64 residuals (at most 3 sweeps); rates, right-hand sides, error estimate,
+        guard and recovery stay FP64.
     """
 
     def __init__(self, net, device, *, drift: float | None = None, photo: bool = False,
-                 variant: tuple[int, int] | None = None, tune: bool = False) -> None:  # fmt: skip
+                 variant: tuple[int, int] | None = None, tune: bool = False, precision: str = "fp64") -> None:  # fmt: skip
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision: 'fp64' or 'mixed' expected, got {precision!r}")
         super().__init__(net, device, drift, photo)
+        self.precision = precision
         with torch.cuda.device(self.dev):
             if variant is None:
                 from . import tune as _tune
@@ -91,7 +98,8 @@ class CudaSolver(Solver):
             raise ValueError(f"variant: cells per workgroup 1, 2, 4 or 8 and a batch of 2-16 expected, got {variant}")
         dev = self.dev.index
         info = _hip.device_info(dev)
-        lay = tab.layout()
+        mixed = self.precision == "mixed"
+        lay = tab.layout(mixed)
         self.lds_call = 8 * lay["SIZE"] * self.cpw
         self.lds_rec = 8 * (lay["SIZE"] + tab.ns * tab.ns + 3 * tab.ns) + 4 * tab.ns
         for what, need in (("call", self.lds_call), ("recovery", self.lds_rec)):
@@ -100,7 +108,7 @@ class CudaSolver(Solver):
                     f"one cell's {what} kernel needs {need} B of LDS per workgroup; the device has "
                     f"{info['lds_per_block']}"
                 )
-        code = _hip.compile_source(_kernels.source(tab, self.cpw, self.drift, self.batch))
+        code = _hip.compile_source(_kernels.source(tab, self.cpw, self.drift, self.batch, mixed))
         self._module = _hip.Module(code, dev)
         self._call = self._module.function("mxc_call")
         self._guard = self._module.function("mxc_guard")
@@ -136,7 +144,7 @@ class CudaSolver(Solver):
             "private_bytes": _hip.attribute(self._call, _hip.ATTR_PRIVATE_BYTES),
             "cells_per_cu": self.max_blocks // cus * self.cpw, "max_blocks": self.max_blocks,
             "rec_blocks": self.rec_blocks, "compute_units": cus, "drift": self.drift, "tables": tab.digest(),
-            "variant": (self.cpw, self.batch),
+            "variant": (self.cpw, self.batch), "precision": self.precision,
         }  # fmt: skip
 
     def launch(

@@ -382,7 +382,8 @@ def generate(solver, sequences: list[dict], log=None) -> dict:
 class LearnedSolver(CudaSolver):
     """CudaSolver with the learned warm start of a trained Surrogate.
 
-    The surrogate's cost estimate orders the queue (costliest first). Its step size starts the cells that come
+    The surrogate's cost estimate orders the queue (costliest first) where no previous call over the same cells
+    measured it. Its step size starts the cells that come
     cold (warm="cold": a caller's h that is not finite or below COLD times dt, as a first call's is; warm="all":
     every cell; warm="none"). Its proposed end state is tried first when propose is set: it is kept only when one
     error-controlled try of the whole dt from it passes and moves it by no more than the tolerance. Every other cell
@@ -397,10 +398,10 @@ class LearnedSolver(CudaSolver):
     COLD = 1e-6
 
     def __init__(self, net, device, surrogate: Surrogate, *, drift=None, photo=False, propose: bool = False,
-                 warm: str = "cold") -> None:  # fmt: skip
+                 warm: str = "cold", precision: str = "fp64") -> None:  # fmt: skip
         if warm not in ("cold", "all", "none"):
             raise ValueError(f"warm: 'cold', 'all' or 'none' expected, got {warm!r}")
-        super().__init__(net, device, drift=drift, photo=photo)
+        super().__init__(net, device, drift=drift, photo=photo, precision=precision)
         self.surrogate, self.propose, self.warm = surrogate, propose, warm
         self.predicted_cost: torch.Tensor | None = None
 
@@ -412,5 +413,8 @@ class LearnedSolver(CudaSolver):
             cold = ~torch.isfinite(h_start) | (h_start < self.COLD * dt)
             h_start.copy_(torch.where(cold, h0, h_start))
         self.predicted_cost = cost
-        order = torch.argsort(cost, descending=True).to(torch.int32)
+        # the queue: the previous call's tries when it had these cells (a measured cost), else the predicted one
+        prev = self.tries
+        key = prev if prev is not None and prev.shape[0] == B else cost
+        order = torch.argsort(key, descending=True).to(torch.int32)
         return order, (xp if self.propose else None)

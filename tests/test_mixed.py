@@ -1,0 +1,64 @@
+"""The mixed-precision form (GPU): FP32 trailing factors with FP64-residual refinement, on both bench networks, against
+the FP64 oracle to the parity bar (species above 1e-20 within relative 1e-2) with the FP64 form's done flags; each
+form's error against the oracle and against each other, and the tries each takes, are printed.
+
+A try whose refinement does not converge is rejected (the step shrinks), so the mixed form keeps the method's
+accuracy at the price of tries: on stiff chemistry at steps of 1e2 s and more, M0's condition number (1e9 to 1e17)
+is beyond what FP32 factors can refine."""
+
+import numpy as np
+import oracle
+import pytest
+import torch
+
+from maxichem_amd import _tables, standin, workloads
+from maxichem_amd.kincuda import CudaSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def t(a):
+    return torch.tensor(np.asarray(a), dtype=torch.float64, device="cuda")
+
+
+def rel(a, b):
+    big = b > 1e-20
+    return float(np.abs(a[big] / b[big] - 1.0).max())
+
+
+def shares(x, atoms):
+    e = np.asarray(x) @ atoms
+    return e / e.sum(1, keepdims=True)
+
+
+@pytest.mark.parametrize("shape", ["NCHO", "SNCHO"])
+def test_mixed_matches_fp64_and_the_oracle(shape):
+    if not torch.cuda.is_available():
+        pytest.skip("this test needs a GPU")
+    net = standin.load(shape)
+    inp = workloads.config1(net.species, len(net.photo), cells=1024, seed=4)
+    orc = oracle.Oracle(_tables.build(net, True))
+    sample = slice(0, 128)
+    with CudaSolver(net, "cuda", photo=True) as f64, CudaSolver(net, "cuda", photo=True, precision="mixed") as mix:
+        assert mix.info["precision"] == "mixed"
+        x, h = inp["x"], inp["h"]
+        calls = [(inp["dt_relax"], np.zeros_like(inp["J"]))] + [(inp["dt"], inp["J"])] * 3
+        tf = tm = 0.0
+        for k, (dt, J) in enumerate(calls):
+            args = (t(x), t(inp["T"]), t(inp["P"]), t(dt), t(h))
+            a = f64.launch(*args, J=t(J))
+            tf += float(f64.tries.sum())
+            b = mix.launch(*args, J=t(J))
+            tm += float(mix.tries.sum())
+            xa, xb = a.x.cpu().numpy(), b.x.cpu().numpy()
+            want = orc.call(x[sample], inp["T"][sample], inp["P"][sample], dt[sample], h[sample], J=J[sample])
+            s_err = np.abs(shares(xb, net.atoms) - shares(xa, net.atoms)).max()
+            print(f"{shape} call {k}: mixed vs FP64 max rel {rel(xb, xa):.2e}, shares {s_err:.1e}, done "
+                  f"{int(b.done.sum())}/{int(a.done.sum())}; against the oracle on 128 cells: FP64 "
+                  f"{rel(xa[sample], want['x']):.2e}, mixed {rel(xb[sample], want['x']):.2e}; tries FP64 "
+                  f"{float(f64.tries.sum()):.0f} mixed {float(mix.tries.sum()):.0f}; recomputed "
+                  f"{mix.fallback['recomputed']}")  # fmt: skip
+            assert torch.equal(a.done, b.done)
+            assert rel(xb[sample], want["x"]) <= 1e-2
+            x, h = xa, a.h.cpu().numpy()  # both continue from the FP64 state
+        print(f"{shape}: tries over the calls FP64 {tf:.0f}, mixed {tm:.0f}")

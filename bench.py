@@ -30,6 +30,9 @@ from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
 FP64_PEAK = 78.6  # TFLOP/s, MI355X FP64 vector
+# HBM bytes the plain FP64 call kernel moves per cell-call on the 69-species workload, from FETCH_SIZE and WRITE_SIZE
+# counters of the timed calls scaled by a 512 MiB copy (profiles/mxc_call_ncho_65536_traffic.md)
+TRAFFIC_PER_CELL_CALL = {"NCHO": 2016}
 
 
 def parse() -> argparse.Namespace:
@@ -41,6 +44,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--network", default="NCHO", choices=("NCHO", "SNCHO"))
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="leave out the CPU baseline (profiling runs)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the timed calls, copy a tensor of this many MiB once: a kernel of known traffic that "
+                         "scales hardware byte counters")
     ap.add_argument("--form", default="all", choices=("plain", "mixed", "learned", "learned_mixed", "all"),
                     help="plain FP64; mixed (FP32 trailing LU, FP64 refinement); learned (the learned warm start, FP64);"
                          " learned_mixed (learned + FP32 LU, the metric's form); all: each from the same start")
@@ -323,7 +329,9 @@ def main() -> int:
         "roofline": {
             "bound": "mfma", "compute": "FP64 (vector FMA; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
             "achieved": main_run["achieved"], "peak": FP64_PEAK, "unit": "TFLOP/s",
-            "frac": main_run["achieved"] / FP64_PEAK, "traffic": None,
+            "frac": main_run["achieved"] / FP64_PEAK,
+            "traffic": (TRAFFIC_PER_CELL_CALL[args.network] * args.cells
+                        if headline == "plain" and args.network in TRAFFIC_PER_CELL_CALL else None),
             "kernel": "mxc_call", "kernel_ms": float(main_run["kernel_ms"].mean()),
             "flop_per_try": work.per_try, "flop_rates_per_cell": work.rates,
             "tries_per_launch": float(np.mean([t.sum() for t in main_run["tries"]])),
@@ -344,6 +352,12 @@ def main() -> int:
             runs[name]["wall"] / len(runs[name]["tries"]))
     if ndata:
         line["forms"]["training_calls"] = ndata
+    if args.copy_mib:
+        src = torch.empty(args.copy_mib << 17, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = src.clone()
+        torch.cuda.synchronize()
+        line["copy_probe_mib"] = args.copy_mib
+        del src, dst
     if not args.no_cpu:
         first = runs["plain"]["first"] if "plain" in runs else main_run["first"]
         first_np = {"x": first.x.cpu().numpy(), "done": first.done.cpu().numpy()}

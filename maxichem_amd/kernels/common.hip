@@ -22,6 +22,7 @@ constexpr double TROE_D = 0.14;
 constexpr int REFUSED_COLD = 1;
 constexpr int REFUSED_BAD_INPUT = 2;
 constexpr int REFUSED_HOT = 3;
+constexpr int SMALL_LEAD = 8;  // at most this many lead rows: their factorization and solves are unrolled
 
 // Orders the LDS accesses of the wavefront's lanes: the wave executes in lockstep and LDS serves one wave's accesses
 // in issue order, so only the compiler must not move loads and stores across this point.
@@ -402,30 +403,70 @@ template <typename T>
 __device__ __forceinline__ void factor_lead(T (&c)[NT], double& dl, double* ws, int lane) {
   double* A21 = ws + L_A21;
   dl = 1.0;
-#pragma unroll 1
-  for (int k = 0; k < K0; k++) {
-    const double dk = 1.0 / ws[k * NS + k];
-    if (lane == k) dl = dk;
-    if (lane > k && lane < K0) ws[lane * NS + k] *= dk;
-    if (lane < NT) A21[lane * K0 + k] *= dk;
-    wave_sync();
-    for (int i = k + 1; i < K0; i++) {
-      const double l = ws[i * NS + k];
-      for (int j = k + 1 + lane; j < NS; j += WAVE) ws[i * NS + j] = fma(-l, ws[k * NS + j], ws[i * NS + j]);
+  if constexpr (K0 <= SMALL_LEAD) {
+    // few lead rows: the steps unrolled, and this lane's row of A21 in registers until the end
+    double a21[K0 > 0 ? K0 : 1];
+#pragma unroll
+    for (int k = 0; k < K0; k++) a21[k] = lane < NT ? A21[lane * K0 + k] : 0.0;
+#pragma unroll
+    for (int k = 0; k < K0; k++) {
+      const double dk = 1.0 / ws[k * NS + k];
+      if (lane == k) dl = dk;
+      if (lane > k && lane < K0) ws[lane * NS + k] *= dk;
+      a21[k] *= dk;
+      wave_sync();
+#pragma unroll
+      for (int i = k + 1; i < K0; i++) {
+        const double l = ws[i * NS + k];
+#pragma unroll
+        for (int j0 = k + 1; j0 < NS; j0 += WAVE) {
+          const int j = j0 + lane;
+          if (j < NS) ws[i * NS + j] = fma(-l, ws[k * NS + j], ws[i * NS + j]);
+        }
+      }
+#pragma unroll
+      for (int j = k + 1; j < K0; j++) a21[j] = fma(-a21[k], ws[k * NS + j], a21[j]);
+      wave_sync();
     }
     if (lane < NT) {
-      const double a = A21[lane * K0 + k];
-      for (int j = k + 1; j < K0; j++) A21[lane * K0 + j] = fma(-a, ws[k * NS + j], A21[lane * K0 + j]);
+#pragma unroll
+      for (int k = 0; k < K0; k++) A21[lane * K0 + k] = a21[k];  // the solves read the multipliers from LDS
+      // the Schur update one lead row at a time (unrolled over k, the compiler would load every row at once)
+#pragma unroll 1
+      for (int k = 0; k < K0; k++) {
+        const double a = A21[lane * K0 + k];
+        const double* uk = ws + k * NS + K0;
+#pragma unroll
+        for (int j = 0; j < NT; j++) c[j] = (T)fma(-a, uk[j], (double)c[j]);
+      }
     }
     wave_sync();
-  }
-  if (lane < NT) {
+  } else {
 #pragma unroll 1
     for (int k = 0; k < K0; k++) {
-      const double a = A21[lane * K0 + k];
-      const double* uk = ws + k * NS + K0;
+      const double dk = 1.0 / ws[k * NS + k];
+      if (lane == k) dl = dk;
+      if (lane > k && lane < K0) ws[lane * NS + k] *= dk;
+      if (lane < NT) A21[lane * K0 + k] *= dk;
+      wave_sync();
+      for (int i = k + 1; i < K0; i++) {
+        const double l = ws[i * NS + k];
+        for (int j = k + 1 + lane; j < NS; j += WAVE) ws[i * NS + j] = fma(-l, ws[k * NS + j], ws[i * NS + j]);
+      }
+      if (lane < NT) {
+        const double a = A21[lane * K0 + k];
+        for (int j = k + 1; j < K0; j++) A21[lane * K0 + j] = fma(-a, ws[k * NS + j], A21[lane * K0 + j]);
+      }
+      wave_sync();
+    }
+    if (lane < NT) {
+#pragma unroll 1
+      for (int k = 0; k < K0; k++) {
+        const double a = A21[lane * K0 + k];
+        const double* uk = ws + k * NS + K0;
 #pragma unroll
-      for (int j = 0; j < NT; j++) c[j] = (T)fma(-a, uk[j], (double)c[j]);
+        for (int j = 0; j < NT; j++) c[j] = (T)fma(-a, uk[j], (double)c[j]);
+      }
     }
   }
 }
@@ -465,7 +506,7 @@ __device__ __forceinline__ void factor(double (&c)[NT], double& dl, double& dt, 
 template <int NV>
 __device__ __forceinline__ void solve_lead_forward(double (&yl)[NV], double (&yt)[NV], double* ws, int lane) {
   const double* A21 = ws + L_A21;
-#pragma unroll 1
+#pragma unroll(K0 <= SMALL_LEAD ? SMALL_LEAD : 1)
   for (int k = 0; k < K0; k++) {  // unit L, lead columns
     const double lk = (lane > k && lane < K0) ? ws[lane * NS + k] : 0.0;
     const double tk = lane < NT ? A21[lane * K0 + k] : 0.0;
@@ -515,7 +556,26 @@ __device__ __forceinline__ void solve_trailing(double (&yt)[NV], const double (&
 template <int NV>
 __device__ __forceinline__ void solve_lead_backward(double (&yl)[NV], const double (&yt)[NV], double dl, double* ws,
                                                     int lane) {
-  if (K0 > 0) {
+  if constexpr (K0 > 0 && K0 <= SMALL_LEAD) {
+    // U of the lead rows, trailing columns: lane t's share of each lead row, summed over the lanes
+#pragma unroll
+    for (int i = 0; i < K0; i++) {
+      const double a = lane < NT ? ws[i * NS + K0 + lane] : 0.0;
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const double sum = wave_sum(a * yt[v]);
+        if (lane == i) yl[v] -= sum;
+      }
+    }
+#pragma unroll
+    for (int k = K0 - 1; k >= 0; k--) {  // U, lead columns
+      const double uk = lane < k ? ws[lane * NS + k] : 0.0;
+#pragma unroll
+      for (int v = 0; v < NV; v++) yl[v] = fma(-uk, readlane(yl[v] * dl, k), yl[v]);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) yl[v] *= dl;
+  } else if (K0 > 0) {
     // U of the lead rows, trailing columns: the trailing solution through LDS (the union region is free between
     // the right-hand sides), a batch of columns at a time
     double* xv = ws + L_UNION;

@@ -44,6 +44,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--network", default="NCHO", choices=("NCHO", "SNCHO"))
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="leave out the CPU baseline (profiling runs)")
+    ap.add_argument("--variant", default=None,
+                    help="the call kernel's variant 'cells per workgroup,batch,waves per SIMD' (default: the tuned or "
+                         "the default one)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the timed calls, copy a tensor of this many MiB once: a kernel of known traffic that "
                          "scales hardware byte counters")
@@ -239,7 +242,8 @@ def main() -> int:
     def g(a):
         return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
 
-    solver = CudaSolver(net, dev, photo=True)
+    variant = tuple(int(q) for q in args.variant.split(",")) if args.variant else None
+    solver = CudaSolver(net, dev, photo=True, variant=variant)
     log(f"solver: {solver.info}")
     x, T, P, h, J = g(d["x"]), g(d["T"]), g(d["P"]), g(d["h"]), g(d["J"])
     dt_relax, dt = g(d["dt_relax"]), g(d["dt"])

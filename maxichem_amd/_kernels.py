@@ -51,14 +51,17 @@ _FLAGS = ("FL_FALLOFF", "FL_SECOND_K0", "FL_THREE_BODY", "FL_TROE", "FL_EFF", "F
           "FL_NF_SHIFT", "FL_NR_SHIFT")  # fmt: skip
 
 
-DEFAULT_BATCH = 8
+DEFAULT_BATCH = 16
 
 
-def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixed: bool = False) -> str:
+def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixed: bool = False, wps: int = 1,
+           rates_global: bool = False) -> str:
     """The network header: every size, offset and setting the kernel files read. cpw: cells (wavefronts) per
     workgroup of the call kernel; batch: the rounds of a lane stream whose words are loaded together; mixed: the
-    mixed-precision form (FP32 trailing factors, FP64 refinement)."""
-    lay = tab.layout(mixed)
+    mixed-precision form (FP32 trailing factors, FP64 refinement); wps: the call kernel's register budget in
+    wavefronts per SIMD (2: at most 256 registers, the excess spilled); rates_global: the rate coefficients in a
+    global scratch slot per wavefront instead of LDS."""
+    lay = tab.layout(mixed, not rates_global)
     nch = len(tab.chunk_words)
     rounds = [w.shape[0] for w in tab.chunk_words]
     offsets = [sum(rounds[:c]) for c in range(nch)]
@@ -89,6 +92,8 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixe
         f"__device__ constexpr int CH_OFF[{nch}] = {{{', '.join(map(str, offsets))}}};",
         f"#define HAS_TROE {int(tab.has_troe)}",
         f"#define MIXED {int(mixed)}",
+        f"#define CALL_WAVES_PER_SIMD {int(wps)}",
+        f"#define RATES_GLOBAL {int(rates_global)}",
         f"#define HAS_EFF {int(tab.has_eff)}",
         "#define DNAN __longlong_as_double(0x7ff8000000000000ll)",
     ]
@@ -103,10 +108,11 @@ def mlp_source() -> str:
     return (KERNEL_DIR / MLP_FILE).read_text()
 
 
-def source(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixed: bool = False) -> str:
+def source(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixed: bool = False, wps: int = 1,
+           rates_global: bool = False) -> str:
     """The whole translation unit hiprtc compiles: the header, then the kernel files in order (mixed.hip after
     common.hip in the mixed-precision form)."""
-    parts = [header(tab, cpw, drift, batch, mixed)]
+    parts = [header(tab, cpw, drift, batch, mixed, wps, rates_global)]
     for name in KERNEL_FILES[:1] + (("mixed.hip",) if mixed else ()) + KERNEL_FILES[1:]:
         parts.append(f"// ---- {name} ----\n" + (KERNEL_DIR / name).read_text())
     return "\n".join(parts)

@@ -165,17 +165,16 @@ class Tables:
         return h.hexdigest()[:16]
 
     # ---- the kernel's LDS layout (doubles per cell) ----
-    def layout(self, mixed: bool = False) -> dict[str, int]:
-        """Offsets (doubles) in a cell's LDS region: LEAD [K0][NS], A21 [NT][K0], n [NS + 1], the rate
-        coefficients KF, KR [NR] (kept for the call), dv [NDV + 1], then a union of what the rhs uses (v [NR + 1],
-        pieces [NPART]) with what the Jacobian and the factorization use after it (the trailing diagonal [NT], staging
-        [NT][CHUNK]); G / RT [NS + 1] sits at its start while the rates are computed. mixed: the mixed-precision
-        form's exchange vectors [2][NS + 1] follow."""
+    def layout(self, mixed: bool = False, rates_lds: bool = True) -> dict[str, int]:
+        """Offsets (doubles) in a cell's LDS region: LEAD [K0][NS], A21 [NT][K0], n [NS + 1], dv [NDV + 1], then a
+        union of what the rhs uses (v [NR + 1], pieces [NPART]) with what the Jacobian and the factorization use after
+        it (the trailing diagonal [NT], staging [NT][CHUNK]); G / RT [NS + 1] sits at its start while the rates are
+        computed. mixed: the mixed-precision form's exchange vectors [2][NS + 1] follow; then the rate coefficients
+        KF, KR [NR] (kept for the call) unless rates_lds is False (they are then in global scratch). Everything the
+        Jacobian streams address lies before the optional parts, so the streams hold for every layout."""
         a21 = self.k0 * self.ns
         nvec = a21 + self.nt * self.k0
-        kf = nvec + self.ns + 1
-        kr = kf + self.nr
-        dv = kr + self.nr
+        dv = nvec + self.ns + 1
         union = dv + self.ndv + 1
         v = union
         part = v + self.nr + 1  # v[NR] = 0: the S v stream's padding term
@@ -185,6 +184,10 @@ class Tables:
         xv = size  # the mixed-precision form's exchange vectors [2][NS + 1]
         if mixed:
             size += 2 * (self.ns + 1)
+        kf = size
+        kr = kf + self.nr
+        if rates_lds:
+            size = kr + self.nr
         size += size & 1  # keep the next cell's region 16-byte aligned
         return {"A21": a21, "NVEC": nvec, "KF": kf, "KR": kr, "DV": dv, "UNION": union, "V": v, "PART": part,
                 "DIAG": diag, "STAGE": stage, "XV": xv, "SIZE": size}

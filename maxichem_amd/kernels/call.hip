@@ -5,16 +5,21 @@
 // K3 = W^-1 (F(N + 2 K1) + (K1 - K2) / h), K4 = W^-1 (F(N + 2 K1 + K3) + (K1 - K2 - 8/3 K3) / h), with
 // W^-1 y = M0^-1 y + c0 (1^T M0^-1 y) M0^-1 w by Sherman-Morrison.
 
-extern "C" __global__ void __launch_bounds__(WAVE* CPW)
+extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_waves_per_eu(CALL_WAVES_PER_SIMD)))
     mxc_call(const Net* __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
              const double* __restrict__ Pa, const double* __restrict__ dta, double* __restrict__ hio,
              double* __restrict__ xout, int* __restrict__ doneo, double* __restrict__ noko,
              int* __restrict__ status, int* __restrict__ count, double* __restrict__ tries,
              const int* __restrict__ order, const double* __restrict__ Jin, int B, double rtol, double atol,
-             int max_steps, const double* __restrict__ xprop) {
+             int max_steps, const double* __restrict__ xprop, double* __restrict__ rscratch) {
   extern __shared__ double lds[];
   const int lane = (int)(threadIdx.x & (WAVE - 1)), wv = (int)(threadIdx.x / WAVE);
   double* ws = lds + wv * L_SIZE;
+#if RATES_GLOBAL
+  double* rk = rscratch + (size_t)((int)blockIdx.x * CPW + wv) * (2 * NR);  // this wave's slot
+#else
+  double* rk = ws + L_KF;
+#endif
   const int waves = (int)gridDim.x * CPW;
   int q = (int)blockIdx.x * CPW + wv;
 #if MIXED
@@ -38,7 +43,7 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
                       isfinite(dt) && dt >= 0.0;
     const int st = Tc < TMIN ? REFUSED_COLD : (Tc > TMAX ? REFUSED_HOT : (fine ? 0 : REFUSED_BAD_INPUT));
     if (lane == 0) status[cell] = st;
-    if (st == 0) cell_rates(ws, Tc, nd, net, Jc, lane);
+    if (st == 0) cell_rates(ws, rk, Tc, nd, net, Jc, lane);
     const double hin = hio[cell];
     const double h0 = isfinite(hin) ? hin : dt;  // a non-finite h acts as dt, and is counted
     if (lane == 0 && !isfinite(hin)) atomicAdd(count + COUNT_NONFINITE_H, 1);
@@ -59,7 +64,7 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
       const double ha = verifying ? dt : fmin(s.h, dt - s.t), igh = 1.0 / (GAMMA * ha), ih = 1.0 / ha;
       // F, w and -Jn at N; a try after a rejection is at the same N and computes the same values again
       double Fl = 0.0, Ft = 0.0, ul = 0.0, ut = 0.0, u0l = 0.0, u0t = 0.0;
-      const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+      const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
       jacobian(c, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
       add_diagonal(c, igh, ws, lane);
       double dl;
@@ -89,7 +94,7 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
       sk = vsum(y1l[0], y1t[0], lane) * c0;
       const double K2l = y1l[0] + sk * Ul, K2t = y1t[0] + sk * Ut;
       double Yl = Nl + 2.0 * K1l, Yt = Nt + 2.0 * K1t;
-      rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+      rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
       y1l[0] = fma(ih, K1l - K2l, Fl);
       y1t[0] = fma(ih, K1t - K2t, Ft);
       MXC_SOLVE(1, y1l, y1t);
@@ -97,7 +102,7 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
       const double K3l = y1l[0] + sk * Ul, K3t = y1t[0] + sk * Ut;
       Yl = Nl + 2.0 * K1l + K3l;
       Yt = Nt + 2.0 * K1t + K3t;
-      rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+      rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
       y1l[0] = fma(ih, fma(-RODAS_C43, K3l, K1l - K2l), Fl);
       y1t[0] = fma(ih, fma(-RODAS_C43, K3t, K1t - K2t), Ft);
       MXC_SOLVE(1, y1l, y1t);

@@ -163,17 +163,18 @@ __device__ __forceinline__ double gibbs_rt(const double* __restrict__ a, double 
 
 __device__ __forceinline__ int species_of(u64 w, int m) { return (int)((w >> (8 * m)) & 0xffull); }
 
-// The rate coefficients of every reaction at the cell's T and nd into KF and KR (LDS), kept there for the call.
+// The rate coefficients of every reaction at the cell's T and nd into rk (kf [NR] then kr [NR]; LDS, or the wave's
+// slot of global scratch), kept there for the call.
 // G / RT (elimination order) goes to the scratch at L_UNION first.
-__device__ __forceinline__ void cell_rates(double* ws, double T, double nd, const Net* net, const double* Jc,
-                                           int lane) {
+__device__ __forceinline__ void cell_rates(double* ws, double* rk, double T, double nd, const Net* net,
+                                           const double* Jc, int lane) {
   net = opaque(net);
   const double* rc = net->rc;
   const int* rf = net->rf;
   const u64* rxe = net->rxe;
   double* g = ws + L_UNION;
-  double* kf = ws + L_KF;
-  double* kr = ws + L_KR;
+  double* kf = rk;
+  double* kr = rk + NR;
   const double lT = log(T), iT = 1.0 / T, lKT = log(KBOLTZ * T / P0BAR);
   const double* cf = net->nasa + (T < NASA_SPLIT_T ? 0 : NS * 9);
   for (int i = lane; i < NS; i += WAVE) g[i] = gibbs_rt(cf + 9 * i, T, lT, iT);
@@ -214,7 +215,7 @@ __device__ __forceinline__ void cell_rates(double* ws, double T, double nd, cons
 // derivatives dv/dn of every live reaction into DV. Returns n_tot = sum Y.
 template <bool JAC>
 __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& Ft, double& u0l, double& u0t,
-                                      const Net* net, double nd, double* ws, int lane) {
+                                      const Net* net, double nd, double* ws, const double* rk, int lane) {
   const double ntot = wave_sum((lane < K0 ? Yl : 0.0) + (lane < NT ? Yt : 0.0));
   const double xf = nd / ntot, ixf = ntot / nd;
   double* n = ws + L_NVEC;
@@ -227,8 +228,8 @@ __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& 
   const int* rf = net->rf;
   const u32* svw = net->svw;
   const u32* rowp = net->rowp;
-  const double* kf = ws + L_KF;
-  const double* kr = ws + L_KR;
+  const double* kf = rk;
+  const double* kr = rk + NR;
   double* V = ws + L_V;
   double* DV = ws + L_DV;
   // unrolled, so that the reaction words of every round are loaded ahead of their use

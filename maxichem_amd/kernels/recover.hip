@@ -114,12 +114,17 @@ extern "C" __global__ void __launch_bounds__(WAVE)
                 double* __restrict__ xout, double* __restrict__ hio, int* __restrict__ doneo,
                 double* __restrict__ noko, int* __restrict__ redo, const int* __restrict__ list,
                 int* __restrict__ count, double* __restrict__ tries, const double* __restrict__ Jin, double rtol,
-                double atol, int max_steps) {
+                double atol, int max_steps, double* __restrict__ rscratch) {
   const int nfl = count[COUNT_RECOMPUTED];
   if ((int)blockIdx.x >= nfl) return;
   extern __shared__ double lds[];
   const int lane = (int)threadIdx.x;
   double* ws = lds;
+#if RATES_GLOBAL
+  double* rk = rscratch + (size_t)blockIdx.x * (2 * NR);  // this workgroup's slot
+#else
+  double* rk = ws + L_KF;
+#endif
   double* W = lds + L_SIZE;
   double* dinv = W + NS * NS;
   double* yv = dinv + NS;
@@ -133,7 +138,7 @@ extern "C" __global__ void __launch_bounds__(WAVE)
     const double* xr = x0 + (size_t)cell * NS;
     const double* Jc = NJ ? Jin + (size_t)cell * NJ : nullptr;
     // a listed cell passed the call kernel's screen, so it is not screened again
-    cell_rates(ws, Tc, nd, net, Jc, lane);
+    cell_rates(ws, rk, Tc, nd, net, Jc, lane);
     const double h0 = isfinite(hin[cell]) ? hin[cell] : dt;
     const double m = fmin(h0, dt);
     int ntry = 0;
@@ -146,7 +151,7 @@ extern "C" __global__ void __launch_bounds__(WAVE)
       while (it < max_steps && s.t < dt * CALL_END) {
         const double ha = fmin(s.h, dt - s.t), igh = 1.0 / (GAMMA * ha), ih = 1.0 / ha;
         double Fl = 0.0, Ft = 0.0, ul = 0.0, ut = 0.0, u0l = 0.0, u0t = 0.0;
-        const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+        const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
         jacobian(c, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
         // W = I / (gamma h) - Jn - w 1^T, dense (recover.cuh:124-130)
         to_lds(yv, ul, ut, lane);
@@ -170,13 +175,13 @@ extern "C" __global__ void __launch_bounds__(WAVE)
         dense_solve(W, pr, dinv, yv, lane);
         from_lds(yv, K2l, K2t, lane);
         double Yl = Nl + 2.0 * K1l, Yt = Nt + 2.0 * K1t;
-        rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+        rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
         to_lds(yv, fma(ih, K1l - K2l, Fl), fma(ih, K1t - K2t, Ft), lane);
         dense_solve(W, pr, dinv, yv, lane);
         from_lds(yv, K3l, K3t, lane);
         Yl = Nl + 2.0 * K1l + K3l;
         Yt = Nt + 2.0 * K1t + K3t;
-        rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+        rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
         to_lds(yv, fma(ih, fma(-RODAS_C43, K3l, K1l - K2l), Fl), fma(ih, fma(-RODAS_C43, K3t, K1t - K2t), Ft), lane);
         dense_solve(W, pr, dinv, yv, lane);
         from_lds(yv, K4l, K4t, lane);

@@ -65,9 +65,11 @@ class CudaSolver(Solver):
         The guard's element-drift allowance (DRIFT if None). Keyword only, as photo.
     photo : bool
         The network's photolysis branches enter the call, which then takes J [B, branches].
-    variant : (int, int), optional
-        The call kernel's cells (wavefronts) per workgroup and stream batch; by default the autotuner's stored
-        choice for this network and device, else (1, 8).
+    variant : (int, int[, int[, int]]), optional
+        The call kernel's cells (wavefronts) per workgroup, stream batch, register budget in wavefronts per SIMD
+        (1, or 2 for at most 256 registers with the excess spilled) and the rate coefficients' place (0: LDS, 1: a
+        global scratch slot per wavefront); by default the autotuner's stored choice for this network and device,
+        else (1, 16, 1, 0).
     tune : bool
         Run the autotuner now (maxichem_amd.tune) and store its choice.
     precision : str
@@ -77,7 +79,7 @@ class CudaSolver(Solver):
     """
 
     def __init__(self, net, device, *, drift: float | None = None, photo: bool = False,
-                 variant: tuple[int, int] | None = None, tune: bool = False, precision: str = "fp64") -> None:  # fmt: skip
+                 variant: tuple[int, ...] | None = None, tune: bool = False, precision: str = "fp64") -> None:  # fmt: skip
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision: 'fp64' or 'mixed' expected, got {precision!r}")
         super().__init__(net, device, drift, photo)
@@ -89,17 +91,22 @@ class CudaSolver(Solver):
                 variant = _tune.choose(net, self.dev, photo, self.drift, run=tune)
             self._setup(net, photo, variant)
 
-    def _setup(self, net, photo: bool, variant: tuple[int, int]) -> None:
+    def _setup(self, net, photo: bool, variant: tuple[int, ...]) -> None:
         tab = _tables.build(net, photo)
         self.tab = tab
         self.ns, self.nr, self.nj = tab.ns, tab.nr, tab.nj
-        self.cpw, self.batch = (int(v) for v in variant)  # cells (wavefronts) per workgroup, stream batch
-        if self.cpw not in (1, 2, 4, 8) or self.batch not in (2, 4, 8, 16):
-            raise ValueError(f"variant: cells per workgroup 1, 2, 4 or 8 and a batch of 2-16 expected, got {variant}")
+        v = tuple(int(q) for q in variant)
+        # cells (wavefronts) per workgroup, stream batch, register budget in wavefronts per SIMD, rates in global
+        # scratch (0: in LDS)
+        self.cpw, self.batch, self.wps, self.rates_global = (v + (1, 0)[len(v) - 2 :])[:4] if 2 <= len(v) <= 4 else (0,) * 4
+        if (self.cpw not in (1, 2, 4, 8) or self.batch not in (2, 4, 8, 16) or self.wps not in (1, 2)
+                or self.rates_global not in (0, 1)):  # fmt: skip
+            raise ValueError(f"variant: cells per workgroup 1, 2, 4 or 8, a batch of 2-16, 1 or 2 wavefronts per SIMD "
+                             f"and rates in global scratch 0 or 1 expected, got {variant}")  # fmt: skip
         dev = self.dev.index
         info = _hip.device_info(dev)
         mixed = self.precision == "mixed"
-        lay = tab.layout(mixed)
+        lay = tab.layout(mixed, not self.rates_global)
         self.lds_call = 8 * lay["SIZE"] * self.cpw
         self.lds_rec = 8 * (lay["SIZE"] + tab.ns * tab.ns + 3 * tab.ns) + 4 * tab.ns
         for what, need in (("call", self.lds_call), ("recovery", self.lds_rec)):
@@ -108,7 +115,8 @@ class CudaSolver(Solver):
                     f"one cell's {what} kernel needs {need} B of LDS per workgroup; the device has "
                     f"{info['lds_per_block']}"
                 )
-        code = _hip.compile_source(_kernels.source(tab, self.cpw, self.drift, self.batch, mixed))
+        code = _hip.compile_source(_kernels.source(tab, self.cpw, self.drift, self.batch, mixed, self.wps,
+                                                   bool(self.rates_global)))
         self._module = _hip.Module(code, dev)
         self._call = self._module.function("mxc_call")
         self._guard = self._module.function("mxc_guard")
@@ -116,6 +124,10 @@ class CudaSolver(Solver):
         cus = info["compute_units"]
         self.max_blocks = cus * max(1, _hip.occupancy(self._call, WAVE * self.cpw, self.lds_call))
         self.rec_blocks = cus * max(1, _hip.occupancy(self._recover, WAVE, self.lds_rec))
+        # each call wavefront's and each recovery workgroup's slot of rate coefficients, when they are not in LDS
+        slots = max(self.max_blocks * self.cpw, self.rec_blocks)
+        self._rscratch = (torch.empty(slots * 2 * tab.nr, dtype=torch.float64, device=self.dev)
+                          if self.rates_global else None)  # fmt: skip
         nasa_e = tab.nasa[:, tab.perm, :]  # the kernels index species in elimination order
         t = {
             "perm": tab.perm.astype(np.int32),
@@ -144,7 +156,7 @@ class CudaSolver(Solver):
             "private_bytes": _hip.attribute(self._call, _hip.ATTR_PRIVATE_BYTES),
             "cells_per_cu": self.max_blocks // cus * self.cpw, "max_blocks": self.max_blocks,
             "rec_blocks": self.rec_blocks, "compute_units": cus, "drift": self.drift, "tables": tab.digest(),
-            "variant": (self.cpw, self.batch), "precision": self.precision,
+            "variant": (self.cpw, self.batch, self.wps, self.rates_global), "precision": self.precision,
         }  # fmt: skip
 
     def launch(
@@ -201,7 +213,7 @@ class CudaSolver(Solver):
         net = p(self._net)
         call_args = [net, p(x), p(T), p(P), p(dt), p(ho), p(xo), p(done), p(nok), p(status), p(self.count),
                      p(self.tries), p(order), p(jx), ctypes.c_int(B), ctypes.c_double(rtol), ctypes.c_double(atol),
-                     ctypes.c_int(max_steps), p(xprop)]  # fmt: skip
+                     ctypes.c_int(max_steps), p(xprop), p(self._rscratch)]  # fmt: skip
         grid = max(1, min(-(-B // self.cpw), self.max_blocks))
         if self.timing is not None:
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -216,7 +228,7 @@ class CudaSolver(Solver):
                     guard_args)  # fmt: skip
         rec_args = [net, p(x), p(T), p(P), p(dt), p(h), p(xo), p(ho), p(done), p(nok), p(redo), p(lst), p(self.count),
                     p(self.tries), p(jx), ctypes.c_double(rtol), ctypes.c_double(atol),
-                    ctypes.c_int(REC_STEPS * max_steps)]  # fmt: skip
+                    ctypes.c_int(REC_STEPS * max_steps), p(self._rscratch)]  # fmt: skip
         _hip.launch(self._recover, max(1, min(B, self.rec_blocks)), WAVE, self.lds_rec, stream, rec_args)
 
     def capture(self, x, T, P, dt, h, *, rtol: float = DEFAULT_RTOL, atol: float = ATOL,

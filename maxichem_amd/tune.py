@@ -1,4 +1,5 @@
-"""The call kernel's autotuner: cells (wavefronts) per workgroup and the stream batch, per network and device.
+"""The call kernel's autotuner: cells (wavefronts) per workgroup, the stream batch, the register budget (wavefronts
+per SIMD) and the rate coefficients' place (LDS or global scratch), per network and device.
 
 Each candidate is compiled, loaded and timed on one synthetic uneven call (solar gas from cold, dt log-uniform
 1e1-1e4 s, half of the cells lit), by HIP events around the call kernel; the fastest is stored in the code cache
@@ -17,8 +18,8 @@ import torch
 
 from . import _hip, _kernels, _tables, workloads
 
-VARIANTS = ((1, 8), (2, 8), (4, 8), (1, 4), (2, 4), (4, 4), (1, 16))
-DEFAULT = (1, _kernels.DEFAULT_BATCH)
+VARIANTS = ((1, 16, 1, 0), (1, 8, 1, 0), (2, 16, 1, 0), (4, 16, 1, 0), (1, 16, 2, 0), (1, 16, 1, 1), (1, 16, 2, 1))
+DEFAULT = (1, _kernels.DEFAULT_BATCH, 1, 0)
 STORE: Path | None = None  # where the choices are kept; None: tune.json in the code cache
 
 
@@ -68,7 +69,7 @@ def tune(net, device, photo: bool, drift: float, cells: int = 8192, seed: int = 
             for _ in range(repeats):
                 s(x, T, P, dtt, h, J=J)
             torch.cuda.synchronize(dev)
-            ms[f"{v[0]},{v[1]}"] = float(np.median([a.elapsed_time(b) for a, b in s.timing]))
+            ms[",".join(map(str, v))] = float(np.median([a.elapsed_time(b) for a, b in s.timing]))
             s.timing = None
     best = min(ms, key=ms.get)
     out = {"best": [int(q) for q in best.split(",")], "ms": ms, "cells": cells}

@@ -16,11 +16,11 @@ extern "C" __global__ void __launch_bounds__(WAVE)
   double* o = out + 8;
   const double nd = Pc / (KBOLTZ * Tc);
   double c[NT];
-  cell_rates(ws, Tc, nd, net, Jc, lane);
+  cell_rates(ws, ws + L_KF, Tc, nd, net, Jc, lane);
   const int* perm = net->perm;
   double Nl = lane < K0 ? xr[perm[lane]] : 0.0, Nt = lane < NT ? xr[perm[K0 + lane]] : 0.0;
   double Fl = 0.0, Ft = 0.0, ul = 0.0, ut = 0.0, u0l = 0.0, u0t = 0.0;
-  const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+  const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, ws + L_KF, lane);
   jacobian(c, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
   auto put = [&](double* dst, double vl, double vt) {
     if (lane < K0) dst[lane] = vl;
@@ -61,7 +61,7 @@ extern "C" __global__ void __launch_bounds__(WAVE)
   const double K2l = y1l[0] + sk * Ul, K2t = y1t[0] + sk * Ut;
   put(v + 2 * NS, K2l, K2t);
   double Yl = Nl + 2.0 * K1l, Yt = Nt + 2.0 * K1t;
-  rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+  rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, ws + L_KF, lane);
   y1l[0] = fma(ih, K1l - K2l, Fl);
   y1t[0] = fma(ih, K1t - K2t, Ft);
   solve<1>(y1l, y1t, c, dl, dtr, ws, lane);
@@ -70,7 +70,7 @@ extern "C" __global__ void __launch_bounds__(WAVE)
   put(v + 3 * NS, K3l, K3t);
   Yl = Nl + 2.0 * K1l + K3l;
   Yt = Nt + 2.0 * K1t + K3t;
-  rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, lane);
+  rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, ws + L_KF, lane);
   y1l[0] = fma(ih, fma(-RODAS_C43, K3l, K1l - K2l), Fl);
   y1t[0] = fma(ih, fma(-RODAS_C43, K3t, K1t - K2t), Ft);
   solve<1>(y1l, y1t, c, dl, dtr, ws, lane);

@@ -72,3 +72,20 @@ def test_streams_decode_to_s_v_and_minus_jn(read_network):
                         dests.append((id(words), int(w) >> T_DEST))
         assert len(dests) == len(set(dests))
         assert terms == tab.jac_terms
+
+
+def test_every_layout_keeps_the_offsets_the_streams_address(read_network):
+    # the Jacobian streams carry LDS offsets built with the default layout: the optional parts (the mixed form's
+    # vectors, the rates in LDS) must not move anything the streams or the rhs address
+    from maxichem_amd import _tables, standin
+
+    for net in (read_network("synthetic"), standin.load("NCHO")):
+        tab = _tables.build(net, True)
+        base = tab.layout()
+        fixed = ("A21", "NVEC", "DV", "UNION", "V", "PART", "DIAG", "STAGE")
+        for mixed in (False, True):
+            for rates in (False, True):
+                lay = tab.layout(mixed, rates)
+                assert all(lay[k] == base[k] for k in fixed)
+                assert lay["SIZE"] >= max(lay["XV"] + (2 * (tab.ns + 1) if mixed else 0),
+                                          (lay["KR"] + tab.nr) if rates else 0)  # fmt: skip

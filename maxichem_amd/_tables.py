@@ -273,8 +273,9 @@ def build(net: VulcanNetwork, photo: bool = False) -> Tables:
         rxe[r] = w
     tab.rxe = rxe
     lay = tab.layout()
-    if lay["SIZE"] >= (1 << T_DEST_BITS):
-        raise ValueError(f"a cell's LDS region of {lay['SIZE']} doubles exceeds the stream's destination field")
+    reach = max(lay["STAGE"] + nt * CHUNK, lay["DIAG"] + nt)  # the furthest LDS offset a stream word addresses
+    if reach >= (1 << T_DEST_BITS):
+        raise ValueError(f"a cell's LDS region of {reach} doubles exceeds the stream's destination field")
     zero = ndv  # dv[ndv] holds 0.0: the padding words' term
 
     def dest(i: int, j: int) -> tuple[int, int]:
@@ -342,6 +343,4 @@ def build(net: VulcanNetwork, photo: bool = False) -> Tables:
                 w.append(word)
         per_lane.append(w)
     tab.sv_words = _pack_rounds(per_lane, nr)  # padding: reaction index nr, whose v slot holds 0
-    if tab.layout()["SIZE"] >= (1 << T_DEST_BITS):
-        raise ValueError(f"a cell's LDS region of {tab.layout()['SIZE']} doubles is too large for the kernel")
     return tab

@@ -63,6 +63,65 @@ int mxc_device_info(int device, int* compute_units, int* lds_per_block, int* wav
  * (_cuda.py:44-49). */
 const char* mxc_last_error(void);
 
+/* ---- A solver without Python, from an export folder ----
+ *
+ * CudaSolver.export(folder) (maxichem_amd/_export.py) writes the call's code object, tables and meta.txt; these
+ * entry points load and run them with the Python call's kernel arguments, so a call gives the same bits. They
+ * replace the reference's C++ caller, maxichem::Solver (cpp/maxichem.hpp:44-92, cpp/maxichem.cpp:200-398); the C++
+ * class maxichem::Solver of include/maxichem.hpp wraps them for libtorch callers. Calls of one solver share its
+ * counts and work buffers: they run one at a time (one stream, or synchronized between streams). */
+
+typedef struct mxc_solver mxc_solver; /* an export loaded on one device */
+
+/* mxc_solver_call's code when the call refused cells (bad input, or T outside the solver's range): the message is
+ * mxc_last_error's, the outputs and counts are written. */
+#define MXC_REFUSED 2
+
+/* Loads the export in folder on device. Every check of meta.txt and the table files (format, argument lists of the
+ * kernels, table sizes) comes before any device work; the device must be the code object's architecture. Replaces
+ * Solver::Solver (cpp/maxichem.cpp:200-263). */
+int mxc_solver_open(const char* folder, int device, mxc_solver** solver);
+
+/* Waits for the device, frees the solver's memory and unloads its code. Replaces Solver::~Solver
+ * (cpp/maxichem.cpp:265-269). */
+int mxc_solver_close(mxc_solver* solver);
+
+/* Species, photolysis branches, elements and count slots. Replaces Solver::ns, nj (cpp/maxichem.hpp:65-66). */
+int mxc_solver_sizes(const mxc_solver* solver, int* ns, int* nj, int* nel, int* count_slots);
+
+/* The call's default rtol, atol and max_steps (the Python call's defaults, maxichem/_constants.py:12-25). Replaces
+ * kDefaultRtol, kDefaultMaxSteps and Solver::default_atol (cpp/maxichem.hpp:33-34,69). */
+int mxc_solver_defaults(const mxc_solver* solver, double* rtol, double* atol, int* max_steps);
+
+/* A name: list 0 species (x's column order), 1 elements (the atoms table's columns), 2 count slots (the queue, the
+ * ten counts of the Python solver's COUNTS, the recovery's queue); NULL out of range. Replaces Solver::species and
+ * elements (cpp/maxichem.hpp:67-68). */
+const char* mxc_solver_name(const mxc_solver* solver, int list, int index);
+
+/* One call on stream (a hipStream_t; NULL: the null stream), all arrays on the solver's device:
+ * x [cells, ns], T, P, dt, h [cells] (float64); J [cells, nj] (float64; NULL without photolysis, required with it);
+ * out: x_out [cells, ns], h_out [cells] (the next step sizes), done [cells] (int32, 1 when the call finished and the
+ * result passed the guard), steps [cells] (accepted steps, float64), flag [cells] (int32 0/1/2 as the Python launch's
+ * flag; NULL: not returned). Returns 0, MXC_REFUSED, or 1 for a form error (before any launch) or a runtime failure.
+ * Synchronizes the stream once (the counts). Replaces Solver::operator() and Solver::refuse
+ * (cpp/maxichem.cpp:271-398) and CudaSolver.launch (maxichem/kincuda.py:230-320). */
+int mxc_solver_call(mxc_solver* solver, void* stream, int cells, const double* x, const double* T, const double* P,
+                    const double* dt, const double* h, const double* J, double rtol, double atol, int max_steps,
+                    double* x_out, double* h_out, int* done, double* steps, int* flag);
+
+/* Host copies of the last call's counts (int [count_slots]) and their sum since mxc_solver_open (long long
+ * [count_slots]); either may be NULL. Replaces Solver::count and total (cpp/maxichem.hpp:72-73). */
+int mxc_solver_counts(const mxc_solver* solver, int* count, long long* total);
+
+/* A host copy of the atoms table, [ns][nel] float64 (network species order). Replaces Solver::atoms
+ * (cpp/maxichem.hpp:71). */
+int mxc_solver_atoms(const mxc_solver* solver, double* atoms);
+
+/* The parameter names mxc_solver_call passes to kernel (mxc_call, mxc_guard or mxc_recover), in order, separated by
+ * spaces, into names (size bytes with the NUL). An export must list the same. Replaces the reference caller's
+ * kKernelArguments (cpp/maxichem.cpp:27-49). */
+int mxc_kernel_arguments(const char* kernel, char* names, size_t size);
+
 #ifdef __cplusplus
 }
 #endif

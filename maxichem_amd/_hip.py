@@ -19,9 +19,9 @@ from ._kernels import ARCH, OPTIONS
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "libmaxichem_amd.so"
-SOURCE = HERE / "csrc" / "maxichem_hip.cpp"
+SOURCES = (HERE / "csrc" / "maxichem_hip.cpp", HERE / "csrc" / "solver.cpp")
 INCLUDE = HERE.parent / "include"
-ABI_VERSION = 1
+ABI_VERSION = 2
 CACHE = Path(os.environ.get("MAXICHEM_AMD_CACHE", HERE / "_cache"))
 
 ATTR_REGISTERS, ATTR_PRIVATE_BYTES, ATTR_STATIC_LDS, ATTR_MAX_THREADS = 0, 1, 2, 3
@@ -32,7 +32,7 @@ def build_library() -> Path:
     cmd = [
         "/opt/rocm/bin/hipcc", f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-fPIC", "-shared",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", f"-I{INCLUDE}",
-        "-o", str(LIB) + ".tmp", str(SOURCE), "-lhiprtc",
+        "-o", str(LIB) + ".tmp", *map(str, SOURCES), "-lhiprtc",
     ]  # fmt: skip
     subprocess.run(cmd, check=True)
     os.replace(str(LIB) + ".tmp", LIB)
@@ -66,6 +66,17 @@ def lib() -> ctypes.CDLL:
         h.mxc_function_attribute.argtypes = [vp, i, ctypes.POINTER(i)]
         h.mxc_device_info.argtypes = [i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
         h.mxc_compiler_version.argtypes = [ctypes.POINTER(i), ctypes.POINTER(i)]
+        ip, d = ctypes.POINTER(i), ctypes.c_double
+        h.mxc_solver_open.argtypes = [ctypes.c_char_p, i, ctypes.POINTER(vp)]
+        h.mxc_solver_close.argtypes = [vp]
+        h.mxc_solver_sizes.argtypes = [vp, ip, ip, ip, ip]
+        h.mxc_solver_defaults.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(d), ip]
+        h.mxc_solver_name.argtypes = [vp, i, i]
+        h.mxc_solver_name.restype = ctypes.c_char_p
+        h.mxc_solver_call.argtypes = [vp, vp, i, vp, vp, vp, vp, vp, vp, d, d, i, vp, vp, vp, vp, vp]
+        h.mxc_solver_counts.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_longlong)]
+        h.mxc_solver_atoms.argtypes = [vp, ctypes.POINTER(d)]
+        h.mxc_kernel_arguments.argtypes = [ctypes.c_char_p, ctypes.c_char_p, sz]
         if h.mxc_version() != ABI_VERSION:
             raise HipError(f"{LIB} has ABI version {h.mxc_version()}, {ABI_VERSION} expected: rebuild it")
         _lib = h

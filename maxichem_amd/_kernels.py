@@ -9,6 +9,8 @@ from __future__ import annotations
 
 from pathlib import Path
 
+import numpy as np
+
 from . import _constants as C
 from ._tables import (
     CHUNK,
@@ -40,6 +42,18 @@ OPTIONS = (
     "-packed-fp32-ops",
 )
 GUARD_THREADS = 256
+KERNEL_NAMES = ("mxc_call", "mxc_guard", "mxc_recover")
+# Each kernel's parameters in order (kernels/*.hip); an export lists them and the C++ loader refuses another list, as
+# the reference's KERNEL_ARGUMENTS (maxichem/_constants.py:148-164) does for its kernels.
+KERNEL_ARGUMENTS = {
+    "mxc_call": ("net", "x0", "Ta", "Pa", "dta", "hio", "xout", "doneo", "noko", "status", "count", "tries", "order",
+                 "Jin", "B", "rtol", "atol", "max_steps", "xprop", "rscratch"),
+    "mxc_guard": ("net", "x0", "xout", "doneo", "noko", "status", "redo", "list", "count", "B", "atol"),
+    "mxc_recover": ("net", "x0", "Ta", "Pa", "dta", "hin", "xout", "hio", "doneo", "noko", "redo", "list", "count",
+                    "tries", "Jin", "rtol", "atol", "max_steps", "rscratch"),
+}  # fmt: skip
+# The kernels' Net struct (kernels/common.hip): one device pointer per table, in this order.
+NET_FIELDS = ("perm", "rc", "rf", "nasa", "rxe", "svw", "rowp", "diagw", "leadw", "chw", "atoms")
 
 # The method's settings the kernels read as constexprs of the same names.
 _SETTINGS = (
@@ -98,6 +112,31 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixe
         "#define DNAN __longlong_as_double(0x7ff8000000000000ll)",
     ]
     return "\n".join(lines) + "\n"
+
+
+def lds_bytes(tab: Tables, cpw: int, mixed: bool = False, rates_global: bool = False) -> tuple[int, int]:
+    """Dynamic LDS of a call workgroup (cpw cells) and of a recovery workgroup (one cell and its dense W)."""
+    size = tab.layout(mixed, not rates_global)["SIZE"]
+    return 8 * size * cpw, 8 * (size + tab.ns * tab.ns + 3 * tab.ns) + 4 * tab.ns
+
+
+def host_tables(tab: Tables) -> dict[str, np.ndarray]:
+    """The tables the kernels read through their Net struct, as contiguous host arrays (species in elimination
+    order where the kernels index them so)."""
+    t = {
+        "perm": tab.perm.astype(np.int32),
+        "rc": tab.rc.astype(np.float64),
+        "rf": tab.rf.astype(np.int32),
+        "nasa": tab.nasa[:, tab.perm, :].astype(np.float64),
+        "rxe": tab.rxe.view(np.int64),
+        "svw": tab.sv_words.ravel().view(np.int32),
+        "rowp": tab.row_pieces.view(np.int32),
+        "diagw": tab.diag_words.ravel().view(np.int32),
+        "leadw": tab.lead_words.ravel().view(np.int32),
+        "chw": np.concatenate([w.ravel() for w in tab.chunk_words]).view(np.int32),
+        "atoms": tab.atoms.astype(np.float64),
+    }
+    return {k: np.ascontiguousarray(v) for k, v in t.items()}
 
 
 MLP_FILE = "mlp.hip"

@@ -17,13 +17,20 @@ struct mxc_module {
 };
 
 namespace {
-
 thread_local std::string last_error;
+}  // namespace
 
+namespace mxc_detail {  // shared with solver.cpp
 int fail(const std::string& what) {
   last_error = what;
   return 1;
 }
+void clear_error() { last_error.clear(); }
+}  // namespace mxc_detail
+
+namespace {
+
+using mxc_detail::fail;
 
 int check(hipError_t e, const char* what) {
   if (e == hipSuccess) return 0;
@@ -39,7 +46,7 @@ int check_rtc(hiprtcResult e, const char* what) {
 
 extern "C" {
 
-int mxc_version(void) { return 1; }
+int mxc_version(void) { return 2; }
 
 int mxc_compiler_version(int* major, int* minor) {
   last_error.clear();

@@ -11,7 +11,6 @@ from __future__ import annotations
 import ctypes
 from typing import NamedTuple
 
-import numpy as np
 import torch
 
 from . import _hip, _kernels, _tables
@@ -94,6 +93,7 @@ class CudaSolver(Solver):
     def _setup(self, net, photo: bool, variant: tuple[int, ...]) -> None:
         tab = _tables.build(net, photo)
         self.tab = tab
+        self._names = (list(net.species), list(net.elements))
         self.ns, self.nr, self.nj = tab.ns, tab.nr, tab.nj
         v = tuple(int(q) for q in variant)
         # cells (wavefronts) per workgroup, stream batch, register budget in wavefronts per SIMD, rates in global
@@ -106,17 +106,15 @@ class CudaSolver(Solver):
         dev = self.dev.index
         info = _hip.device_info(dev)
         mixed = self.precision == "mixed"
-        lay = tab.layout(mixed, not self.rates_global)
-        self.lds_call = 8 * lay["SIZE"] * self.cpw
-        self.lds_rec = 8 * (lay["SIZE"] + tab.ns * tab.ns + 3 * tab.ns) + 4 * tab.ns
+        self.lds_call, self.lds_rec = _kernels.lds_bytes(tab, self.cpw, mixed, bool(self.rates_global))
         for what, need in (("call", self.lds_call), ("recovery", self.lds_rec)):
             if need > info["lds_per_block"]:
                 raise ValueError(
                     f"one cell's {what} kernel needs {need} B of LDS per workgroup; the device has "
                     f"{info['lds_per_block']}"
                 )
-        code = _hip.compile_source(_kernels.source(tab, self.cpw, self.drift, self.batch, mixed, self.wps,
-                                                   bool(self.rates_global)))
+        self._source = _kernels.source(tab, self.cpw, self.drift, self.batch, mixed, self.wps, bool(self.rates_global))
+        self._code = code = _hip.compile_source(self._source)
         self._module = _hip.Module(code, dev)
         self._call = self._module.function("mxc_call")
         self._guard = self._module.function("mxc_guard")
@@ -128,24 +126,10 @@ class CudaSolver(Solver):
         slots = max(self.max_blocks * self.cpw, self.rec_blocks)
         self._rscratch = (torch.empty(slots * 2 * tab.nr, dtype=torch.float64, device=self.dev)
                           if self.rates_global else None)  # fmt: skip
-        nasa_e = tab.nasa[:, tab.perm, :]  # the kernels index species in elimination order
-        t = {
-            "perm": tab.perm.astype(np.int32),
-            "rc": tab.rc.astype(np.float64),
-            "rf": tab.rf.astype(np.int32),
-            "nasa": np.ascontiguousarray(nasa_e, np.float64),
-            "rxe": tab.rxe.view(np.int64),
-            "svw": tab.sv_words.ravel().view(np.int32),
-            "rowp": tab.row_pieces.view(np.int32),
-            "diagw": tab.diag_words.ravel().view(np.int32),
-            "leadw": tab.lead_words.ravel().view(np.int32),
-            "chw": np.concatenate([w.ravel() for w in tab.chunk_words]).view(np.int32),
-            "atoms": tab.atoms.astype(np.float64),
-        }
-        self._tables = {k: torch.from_numpy(np.ascontiguousarray(v)).to(self.dev) for k, v in t.items()}
+        self._tables = {k: torch.from_numpy(v).to(self.dev) for k, v in _kernels.host_tables(tab).items()}
         # the kernels' Net struct: the tables' device pointers, in its field order
-        fields = ("perm", "rc", "rf", "nasa", "rxe", "svw", "rowp", "diagw", "leadw", "chw", "atoms")
-        self._net = torch.tensor([self._tables[k].data_ptr() for k in fields], dtype=torch.int64, device=self.dev)
+        self._net = torch.tensor([self._tables[k].data_ptr() for k in _kernels.NET_FIELDS], dtype=torch.int64,
+                                 device=self.dev)  # fmt: skip
         self.tries: torch.Tensor | None = None
         # a list to collect (start, end) events of each call kernel launch on the call's stream; None: not recorded
         self.timing: list | None = None
@@ -230,6 +214,17 @@ class CudaSolver(Solver):
                     p(self.tries), p(jx), ctypes.c_double(rtol), ctypes.c_double(atol),
                     ctypes.c_int(REC_STEPS * max_steps), p(self._rscratch)]  # fmt: skip
         _hip.launch(self._recover, max(1, min(B, self.rec_blocks)), WAVE, self.lds_rec, stream, rec_args)
+
+    def export(self, folder) -> None:
+        """Write what a caller without Python needs to run this solver's kernels (maxichem/kincuda.py:363-373): the
+        code object, its source, the tables and meta.txt, which mxc_solver_open and maxichem::Solver
+        (include/maxichem.hpp) load. The code object is for gfx950."""
+        self._require_open()
+        from . import _export
+
+        _export.write(folder, tab=self.tab, source=self._source, code=self._code, species=self._names[0],
+                      elements=self._names[1], cpw=self.cpw, batch=self.batch, wps=self.wps,
+                      rates_global=bool(self.rates_global), precision=self.precision, drift=self.drift)  # fmt: skip
 
     def capture(self, x, T, P, dt, h, *, rtol: float = DEFAULT_RTOL, atol: float = ATOL,
                 max_steps: int = DEFAULT_MAX_STEPS, J=None) -> "CallGraph":  # fmt: skip

@@ -56,9 +56,10 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--train-cells", type=int, default=16384, help="cells per training sequence")
     ap.add_argument("--train-sequences", type=int, default=6, help="training sequences (seeds 101..)")
     ap.add_argument("--train-epochs", type=float, default=4.0)
-    ap.add_argument("--workload", default="calls", choices=("calls", "uneven"),
+    ap.add_argument("--workload", default="calls", choices=("calls", "uneven", "coupling"),
                     help="calls: the 65,536-cell call sequence (the metric's); uneven: 262,144 cells, one call each "
-                         "from cold, half lit, dt log-uniform 1e1-1e6 s, by graph replay with the tuned variant")
+                         "from cold, half lit, dt log-uniform 1e1-1e6 s, by graph replay with the tuned variant; "
+                         "coupling: the GCM-side conversion kernels on 65,536 cells, against HBM bandwidth")
     return ap.parse_args()
 
 
@@ -217,6 +218,86 @@ def run_uneven(args, net, dev) -> int:
     return 0
 
 
+HBM_PEAK = 8000.0  # GB/s, MI355X HBM3E
+
+
+def run_coupling(args, net, dev) -> int:
+    """The GCM-side conversions (maxichem_amd.coupling.composition) on one cell batch: each kernel's time by HIP
+    events on the stream it runs on, against its algorithmic bytes (each [cells, S] array read or written once)."""
+    import numpy as np
+    import torch
+
+    from maxichem_amd import workloads
+    from maxichem_amd.coupling import composition as cc
+
+    cells, S, E = args.cells, len(net.species), len(net.elements)
+    d = workloads.config1(net.species, len(net.photo), cells)
+    rng = np.random.default_rng(5)
+    mw = torch.from_numpy(cc.molar_masses(net.atoms, net.elements).numpy()).to(dev)
+    atoms = torch.from_numpy(np.asarray(net.atoms, np.float64)).to(dev)
+    x = torch.from_numpy(d["x"]).to(dev)
+    mass = torch.from_numpy(rng.uniform(0.5, 1.5, cells)).to(dev)
+    cell_mass = torch.from_numpy(rng.uniform(1e3, 1e6, cells)).to(dev)
+    q = cc.mole_to_mass(x, mw, mass)
+    kernels = {  # name: (call, algorithmic bytes per call)
+        "mxc_to_mole_f64": (lambda: cc.mass_to_mole(q, mw), 16 * cells * S),
+        "mxc_to_mass_f64": (lambda: cc.mole_to_mass(x, mw, mass), 16 * cells * S + 8 * cells),
+        "mxc_element_moles_f64": (lambda: cc.element_moles(q, mw, atoms), 8 * cells * (S + E)),
+        "mxc_element_moles_f64 + mxc_column_sums_f64 (tracer_totals)": (
+            lambda: cc.tracer_totals(q, cell_mass, mw, atoms), 8 * cells * (S + 1)),
+    }
+    kernels["copy of [cells, S] (torch clone: the same bytes as mxc_to_mole)"] = (lambda: q.clone(), 16 * cells * S)
+    out = {}
+    for name, (call, nbytes) in kernels.items():
+        for _ in range(max(1, args.warmup)):
+            call()
+        torch.cuda.synchronize()
+        ms, walls = [], []
+        for _ in range(args.steps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            a.record()
+            call()  # the conversions read their flags back: such a call ends synchronized
+            b.record()
+            b.synchronize()
+            walls.append(time.perf_counter() - t0)
+            ms.append(a.elapsed_time(b))
+        gbs = nbytes / (float(np.median(ms)) * 1e-3) / 1e9
+        out[name] = {"ms": float(np.median(ms)), "wall_ms": float(np.median(walls)) * 1e3, "bytes": nbytes,
+                     "GB/s": gbs, "frac": gbs / HBM_PEAK}  # fmt: skip
+        log(f"{name}: {out[name]['ms'] * 1e3:.1f} us between events (flags fill, kernel, flags read), "
+            f"{out[name]['wall_ms'] * 1e3:.1f} us wall; {gbs:.0f} GB/s = {gbs / HBM_PEAK:.1%} of HBM")
+    # the roofline's kernel alone between events on its stream (the calls above include the flags' fill and read)
+    x_out, flags = torch.empty_like(q), cc._new_flags(dev)
+    stream = torch.cuda.current_stream(dev)
+    ka = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    cc._launch_to_mole(q, mw, x_out, flags, False)
+    ka[0].record(stream)
+    for k in range(args.steps):
+        cc._launch_to_mole(q, mw, x_out, flags, False)
+        ka[k + 1].record(stream)
+    ka[-1].synchronize()
+    kernel_ms = float(np.mean([ka[k].elapsed_time(ka[k + 1]) for k in range(args.steps)]))
+    head = dict(out["mxc_to_mole_f64"])
+    head["GB/s"] = 16 * cells * S / (kernel_ms * 1e-3) / 1e9
+    head["frac"] = head["GB/s"] / HBM_PEAK
+    log(f"mxc_to_mole_f64 alone: {kernel_ms * 1e3:.1f} us per launch, {head['GB/s']:.0f} GB/s = {head['frac']:.1%} "
+        "of HBM")  # fmt: skip
+    line = {
+        "metric": "GB/s (GCM-side conversions, NCHO, 64K cells)", "value": head["GB/s"], "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": kernel_ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"coupling: {args.network} {cells} cells x {S} species, mass to mole fractions and "
+                               "back, element moles, totals"},
+        "kernels": out,
+        "roofline": {"bound": "hbm", "achieved": head["GB/s"], "peak": HBM_PEAK, "unit": "GB/s",
+                     "frac": head["frac"], "traffic": None, "kernel": "mxc_to_mole_f64"},
+        "cpu_baseline": None,
+    }  # fmt: skip
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -235,6 +316,8 @@ def main() -> int:
     nj = len(net.photo)
     if args.workload == "uneven":
         return run_uneven(args, net, dev)
+    if args.workload == "coupling":
+        return run_coupling(args, net, dev)
     d = workloads.config1(net.species, nj, args.cells)
     log(f"network {args.network}: {len(net.species)} species, {len(net.reac)} thermal, {nj} photolysis; "
         f"{args.cells} cells")  # fmt: skip

@@ -22,6 +22,7 @@ constexpr double TROE_D = 0.14;
 constexpr int REFUSED_COLD = 1;
 constexpr int REFUSED_BAD_INPUT = 2;
 constexpr int REFUSED_HOT = 3;
+constexpr int BROADCAST_WINDOW = 32;  // v_readlane broadcasts of a pivot row in flight ahead of their updates
 constexpr int SMALL_LEAD = 8;  // at most this many lead rows: their factorization and solves are unrolled
 
 // Orders the LDS accesses of the wavefront's lanes: the wave executes in lockstep and LDS serves one wave's accesses
@@ -96,6 +97,9 @@ __device__ __forceinline__ P opaque(P p) {
 // them): it keeps a step's values from being computed early or late, which would stretch their live ranges.
 __device__ __forceinline__ void pin(double& a, double& b) { asm volatile("" : "+v"(a), "+v"(b)); }
 __device__ __forceinline__ void pin(double& a) { asm volatile("" : "+v"(a)); }
+// a's next use waits for b (no instruction; an ordering the compiler keeps)
+__device__ __forceinline__ void tie(double& a, double b) { asm volatile("" : "+v"(a) : "v"(b)); }
+__device__ __forceinline__ void tie(float& a, float b) { asm volatile("" : "+v"(a) : "v"(b)); }
 
 __device__ __forceinline__ bool wave_all(bool p) { return __ballot(!p) == 0ull; }
 
@@ -398,6 +402,20 @@ __device__ __forceinline__ void add_diagonal(T (&c)[NT], double igh, double* ws,
   wave_sync();
 }
 
+// One lead row's Schur update of the trailing block: c_j -= a U_kj, a = this lane's L entry in column k. The row's
+// trailing entries come one per lane from a single LDS read and reach every lane by v_readlane (a same-address LDS
+// read per entry would wait out the LDS latency once per pair of columns), at most BROADCAST_WINDOW ahead of their
+// updates.
+template <typename T>
+__device__ __forceinline__ void schur_row(T (&c)[NT], double a, const double* ws, int k, int lane) {
+  const double u = lane < NT ? ws[k * NS + K0 + lane] : 0.0;
+#pragma unroll
+  for (int j = 0; j < NT; j++) {
+    if (j >= BROADCAST_WINDOW) tie(c[j], c[j - BROADCAST_WINDOW]);
+    c[j] = (T)fma(-a, readlane(u, j), (double)c[j]);
+  }
+}
+
 // LU of the lead columns (right-looking, dense in LDS) and the Schur update of the trailing block (in T). dl: 1 / U_ii
 // of lead row lane.
 template <typename T>
@@ -432,15 +450,10 @@ __device__ __forceinline__ void factor_lead(T (&c)[NT], double& dl, double* ws, 
     if (lane < NT) {
 #pragma unroll
       for (int k = 0; k < K0; k++) A21[lane * K0 + k] = a21[k];  // the solves read the multipliers from LDS
-      // the Schur update one lead row at a time (unrolled over k, the compiler would load every row at once)
-#pragma unroll 1
-      for (int k = 0; k < K0; k++) {
-        const double a = A21[lane * K0 + k];
-        const double* uk = ws + k * NS + K0;
-#pragma unroll
-        for (int j = 0; j < NT; j++) c[j] = (T)fma(-a, uk[j], (double)c[j]);
-      }
     }
+    // the Schur update one lead row at a time (unrolled over k, the compiler would load every row at once)
+#pragma unroll 1
+    for (int k = 0; k < K0; k++) schur_row(c, lane < NT ? A21[lane * K0 + k] : 0.0, ws, k, lane);
     wave_sync();
   } else {
 #pragma unroll 1
@@ -460,15 +473,8 @@ __device__ __forceinline__ void factor_lead(T (&c)[NT], double& dl, double* ws, 
       }
       wave_sync();
     }
-    if (lane < NT) {
 #pragma unroll 1
-      for (int k = 0; k < K0; k++) {
-        const double a = A21[lane * K0 + k];
-        const double* uk = ws + k * NS + K0;
-#pragma unroll
-        for (int j = 0; j < NT; j++) c[j] = (T)fma(-a, uk[j], (double)c[j]);
-      }
-    }
+    for (int k = 0; k < K0; k++) schur_row(c, lane < NT ? A21[lane * K0 + k] : 0.0, ws, k, lane);
   }
 }
 
@@ -485,10 +491,16 @@ __device__ __forceinline__ void factor_trailing(double (&c)[NT], double& dt, dou
     if (lane == 0) dinv[k] = dk;
     // column k is scaled by 1 / U_kk off the diagonal: below the pivot it becomes the multiplier L_ik, above it
     // U_ik / U_kk, the unit upper factor the backward solve uses (U = U' diag(U_kk))
-    if (lane != k) c[k] *= dk;
-    const double l = lane > k ? c[k] : 0.0;
+    const int ln = lane_after(lane, dk);  // the step's lane masks made here, not hoisted and spilled for every step
+    if (ln != k) c[k] *= dk;
+    const double l = ln > k ? c[k] : 0.0;
 #pragma unroll
-    for (int j = k + 1; j < NT; j++) c[j] = fma(-l, readlane(c[j], k), c[j]);
+    for (int j = k + 1; j < NT; j++) {
+      // at most BROADCAST_WINDOW of row k's broadcasts ahead of their updates: hoisted further, they would hold up
+      // to 126 scalar registers and spill them to vector lanes
+      if (j >= k + 1 + BROADCAST_WINDOW) tie(c[j], c[j - BROADCAST_WINDOW]);
+      c[j] = fma(-l, readlane(c[j], k), c[j]);
+    }
   }
   wave_sync();
   dt = lane < NT ? dinv[lane] : 1.0;

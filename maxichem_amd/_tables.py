@@ -168,8 +168,8 @@ class Tables:
     def layout(self, mixed: bool = False, rates_lds: bool = True) -> dict[str, int]:
         """Offsets (doubles) in a cell's LDS region: LEAD [K0][NS], A21 [NT][K0], n [NS + 1], dv [NDV + 1], then a
         union of what the rhs uses (v [NR + 1], pieces [NPART]) with what the Jacobian and the factorization use after
-        it (the trailing diagonal [NT], staging [NT][CHUNK]); G / RT [NS + 1] sits at its start while the rates are
-        computed. mixed: the mixed-precision form's exchange vectors [2][NS + 1] follow; then the rate coefficients
+        it (the trailing diagonal [NT], staging [CHUNK][NT], column-major so that a lane's writes and reads of its row
+        fall in distinct banks); G / RT [NS + 1] sits at its start while the rates are computed. mixed: the mixed-precision form's exchange vectors [2][NS + 1] follow; then the rate coefficients
         KF, KR [NR] (kept for the call) unless rates_lds is False (they are then in global scratch). Everything the
         Jacobian streams address lies before the optional parts, so the streams hold for every layout."""
         a21 = self.k0 * self.ns
@@ -291,7 +291,7 @@ def build(net: VulcanNetwork, photo: bool = False) -> Tables:
         if j < k0:
             return -1, lay["A21"] + t * k0 + j
         c = (j - k0) // CHUNK
-        return c, lay["STAGE"] + t * CHUNK + (j - k0) % CHUNK
+        return c, lay["STAGE"] + ((j - k0) % CHUNK) * nt + t  # column-major: a column's rows are consecutive
 
     groups: dict[int, list[tuple[int, list[tuple[int, int]]]]] = {}
     for (p, q), terms in sorted(entries.items(), key=lambda e: (pos[e[0][0]], pos[e[0][1]])):

@@ -349,7 +349,18 @@ __device__ __forceinline__ void jacobian(T (&c)[NT], double Nl, double Nt, doubl
   // -Jn N in FP64, from the entries before the trailing block takes them in T: the lead rows and the lead columns
   // of the trailing rows through the exchange vector, then the trailing columns chunk by chunk, N_j from lane j
   double ml = 0.0, mt = 0.0;
-  if (K0 > 0) {
+  if constexpr (K0 > 0 && K0 <= SMALL_LEAD) {
+    // few lead rows: each row times N as this lane's share (its own N entries), summed over the lanes; the trailing
+    // rows' lead columns times N with N_k by v_readlane
+#pragma unroll
+    for (int i = 0; i < K0; i++) {
+      const double part = (lane < K0 ? ws[i * NS + lane] * Nl : 0.0) + (lane < NT ? ws[i * NS + K0 + lane] * Nt : 0.0);
+      const double sum = wave_sum(part);
+      if (lane == i) ml = sum;
+    }
+#pragma unroll
+    for (int k = 0; k < K0; k++) mt = fma(lane < NT ? ws[L_A21 + lane * K0 + k] : 0.0, readlane(Nl, k), mt);
+  } else if (K0 > 0) {
 #pragma unroll 1
     for (int j0 = 0; j0 < NS; j0 += STREAM_BATCH) {  // the lead rows, a batch of columns at a time
       double a[STREAM_BATCH], nv[STREAM_BATCH];
@@ -371,9 +382,9 @@ __device__ __forceinline__ void jacobian(T (&c)[NT], double Nl, double Nt, doubl
   for (int ch = 0; ch < NCH; ch++) {
     if (lane < NT) {
 #pragma unroll
-      for (int j = 0; j < CHUNK; j++) stage[lane * CHUNK + j] = 0.0;
+      for (int j = 0; j < CHUNK; j++) stage[j * NT + lane] = 0.0;
       // the chunk streams hold no diagonal entry: a trailing row's own diagonal goes in by its lane
-      if (lane / CHUNK == ch) stage[lane * CHUNK + lane % CHUNK] = cd;
+      if (lane / CHUNK == ch) stage[(lane % CHUNK) * NT + lane] = cd;
     }
     wave_sync();
     jac_stream(opaque(net)->chw + CH_OFF[ch] * WAVE, CH_ROUNDS[ch], DV, ws, lane);
@@ -381,7 +392,7 @@ __device__ __forceinline__ void jacobian(T (&c)[NT], double Nl, double Nt, doubl
 #pragma unroll
     for (int j = 0; j < CHUNK; j++)
       if (ch * CHUNK + j < NT) {
-        const double v = lane < NT ? stage[lane * CHUNK + j] : 0.0;
+        const double v = lane < NT ? stage[j * NT + lane] : 0.0;
         c[ch * CHUNK + j] = (T)v;
         mt = fma(v, readlane(Nt, ch * CHUNK + j), mt);
       }

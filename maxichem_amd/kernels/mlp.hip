@@ -143,17 +143,33 @@ extern "C" __global__ void __launch_bounds__(256)
   if (e < n) dZ[e] = f2bf(dA[e] * gelu_grad(Z[e]));
 }
 
-// g[c] = sum over rows of D[r, c] (BF16 D [rows, cols]): a block of 256 threads per 64 columns, four row phases.
+// Column sums of D [rows, cols] (BF16), in two fixed-order passes: mlp_column_sum adds the rows of one block of
+// rows_per_block rows for 64 columns (4 row phases, then the phases) into partial [row blocks, cols]; mlp_column_total
+// adds the row blocks. Block b of the first pass takes columns 64 (b % col_blocks) on, rows rows_per_block
+// (b / col_blocks) on.
 extern "C" __global__ void __launch_bounds__(256)
-    mlp_column_sum(const bf16_t* __restrict__ D, int rows, int cols, float* __restrict__ g) {
+    mlp_column_sum(const bf16_t* __restrict__ D, int rows, int cols, int rows_per_block, int col_blocks,
+                   float* __restrict__ partial) {
   __shared__ float part[4][64];
-  const int c = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63), ph = (int)(threadIdx.x >> 6);
+  const int cb = (int)blockIdx.x % col_blocks, rb = (int)blockIdx.x / col_blocks;
+  const int c = cb * 64 + (int)(threadIdx.x & 63), ph = (int)(threadIdx.x >> 6);
+  const int r0 = rb * rows_per_block, r1 = min(rows, r0 + rows_per_block);
   float s = 0.0f;
   if (c < cols)
-    for (int r = ph; r < rows; r += 4) s += bf2f(D[(long)r * cols + c]);
+    for (int r = r0 + ph; r < r1; r += 4) s += bf2f(D[(long)r * cols + c]);
   part[ph][threadIdx.x & 63] = s;
   __syncthreads();
-  if (ph == 0 && c < cols) g[c] = (part[0][threadIdx.x] + part[1][threadIdx.x]) + (part[2][threadIdx.x] + part[3][threadIdx.x]);
+  if (ph == 0 && c < cols)
+    partial[(long)rb * cols + c] = (part[0][threadIdx.x] + part[1][threadIdx.x]) + (part[2][threadIdx.x] + part[3][threadIdx.x]);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+    mlp_column_total(const float* __restrict__ partial, int blocks, int cols, float* __restrict__ g) {
+  const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= cols) return;
+  float s = 0.0f;
+  for (int b = 0; b < blocks; b++) s += partial[(long)b * cols + c];
+  g[c] = s;
 }
 
 // One Adam step on n parameters (FP32 master copy p, moments m and v, gradient g), with the BF16 copy pb for the
@@ -229,7 +245,8 @@ extern "C" __global__ void __launch_bounds__(256)
 // The predictions of one cell from the network's output row Y [rows, ldy] (standardised by mu and sd):
 // h0 = exp(ln h) within [hmin, dt]; cost = exp(ln(1 + steps)) - 1; and the proposal xp = x 10^dlog10x, scaled to sum 1
 // and then moved onto the cell's element shares: x_i exp(sum_e lambda_e a_ie) with lambda from Newton steps on
-// the shares (atoms [ns, nel]); a proposal that does not settle within the steps gets xp[0] = NaN (no proposal).
+// the shares (atoms [ns, nel]); a proposal that does not settle within the steps gets xp[0] = NaN (no proposal). With
+// xp null only h0 and cost are made.
 extern "C" __global__ void __launch_bounds__(256)
     mlp_decode(const float* __restrict__ Y, int ldy, const float* __restrict__ mu, const float* __restrict__ sd,
                const double* __restrict__ x, const double* __restrict__ dt, const double* __restrict__ atoms, int cells,
@@ -241,6 +258,7 @@ extern "C" __global__ void __launch_bounds__(256)
   const double lh = (double)y[0] * sd[0] + mu[0];
   h0[r] = fmin(fmax(exp(lh), hmin), fmax(dt[r], hmin));
   cost[r] = expm1(fmax((double)y[1] * sd[1] + mu[1], 0.0));
+  if (xp == nullptr) return;
   const double* xr = x + (long)r * ns;
   double* o = xp + (long)r * ns;
   constexpr int MAXEL = 8;

@@ -77,7 +77,7 @@ class MLP:
         self._mod = module(self.dev.index if self.dev.index is not None else torch.cuda.current_device())
         self._fn = {name: self._mod.function(name) for name in (
             "mlp_gemm_f32", "mlp_gemm_bias_gelu", "mlp_gemm_bias", "mlp_loss", "mlp_gelu_backward",
-            "mlp_column_sum", "mlp_adam", "mlp_to_bf16")}  # fmt: skip
+            "mlp_column_sum", "mlp_column_total", "mlp_adam", "mlp_to_bf16")}  # fmt: skip
         self.loss = torch.zeros(1, dtype=torch.float32, device=self.dev)
         self._sync_bf16()
 
@@ -141,8 +141,14 @@ class MLP:
             x = self._acts[l]
             # dW = dZ^T X: (n, m) of dZ at n + m N; (m, k) of X at m K + k
             self._gemm("mlp_gemm_f32", N, K, M, dz, 1, N, x, K, 1, [p(ly.gw), ctypes.c_long(K), ctypes.c_float(1.0)])
-            _hip.launch(self._fn["mlp_column_sum"], N // TILE, 256, 0, self._stream(),
-                        [p(dz), ctypes.c_int(M), ctypes.c_int(N), p(ly.gb)])  # fmt: skip
+            # the bias gradient: column sums of dZ over blocks of rows, then over the blocks
+            rows_per_block = 256
+            rb = -(-M // rows_per_block)
+            partial = torch.empty(rb, N, dtype=torch.float32, device=self.dev)
+            _hip.launch(self._fn["mlp_column_sum"], (N // TILE) * rb, 256, 0, self._stream(),
+                        [p(dz), ctypes.c_int(M), ctypes.c_int(N), ctypes.c_int(rows_per_block), ctypes.c_int(N // TILE),
+                         p(partial)])  # fmt: skip
+            self._elementwise("mlp_column_total", N, [p(partial), ctypes.c_int(rb), ctypes.c_int(N), p(ly.gb)])
             if l > 0:
                 da = torch.empty(M, K, dtype=torch.float32, device=self.dev)
                 # dA = dZ W: (m, n) of dZ at m N + n; (n, k) of W at n K + k
@@ -300,16 +306,17 @@ class Surrogate:
         return out
 
     # ---- predictions ----
-    def predict(self, x, T, P, dt, J, hmin: float):
-        """(h0 [B], cost [B], proposals [B, NS]) of the cells: h0 the warm-start step, cost the expected accepted
-        steps, a proposal's first entry NaN where its element shares could not be matched."""
+    def predict(self, x, T, P, dt, J, hmin: float, proposals: bool = True):
+        """(h0 [B], cost [B], proposals [B, NS] or None) of the cells: h0 the warm-start step, cost the expected
+        accepted steps, a proposal's first entry NaN where its element shares could not be matched; proposals=False
+        leaves them out (and their projection onto the element shares, most of the decoding's work)."""
         B = x.shape[0]
         rows = pad(B)
         X = self.features(x, T, P, dt, J, rows)
         Y = self.mlp.forward(X)
         h0 = torch.empty(B, dtype=torch.float64, device=self.dev)
         cost = torch.empty_like(h0)
-        xp = torch.empty(B, self.ns, dtype=torch.float64, device=self.dev)
+        xp = torch.empty(B, self.ns, dtype=torch.float64, device=self.dev) if proposals else None
         p = _hip.ptr
         _hip.launch(self._fn["mlp_decode"], max(1, -(-B // 256)), 256, 0, self.mlp._stream(),
                     [p(Y), ctypes.c_int(Y.shape[1]), p(self.mu_out), p(self.sd_out), p(x), p(dt), p(self.atoms),
@@ -406,7 +413,7 @@ class LearnedSolver(CudaSolver):
         self.predicted_cost: torch.Tensor | None = None
 
     def _plan(self, x, T, P, dt, h_start, jx, B):
-        h0, cost, xp = self.surrogate.predict(x, T, P, dt, jx, H_MIN)
+        h0, cost, xp = self.surrogate.predict(x, T, P, dt, jx, H_MIN, proposals=self.propose)
         if self.warm == "all":
             h_start.copy_(h0)
         elif self.warm == "cold":

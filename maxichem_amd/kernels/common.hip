@@ -70,6 +70,14 @@ __device__ __forceinline__ double wave_max(double v) {
   v = fmax(v, dpp<DPP_MIRROR>(v));
   return fmax(fmax(readlane(v, 0), readlane(v, 16)), fmax(readlane(v, 32), readlane(v, 48)));
 }
+__device__ __forceinline__ int wave_min(int v) {
+  v = min(v, __builtin_amdgcn_update_dpp(0, v, DPP_XOR1, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(0, v, DPP_XOR2, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(0, v, DPP_HALF_MIRROR, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(0, v, DPP_MIRROR, 0xf, 0xf, false));
+  return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+             min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
 
 // The lane index as an opaque value, taken inside each step of an unrolled loop so that the step's lane masks are
 // computed there and not all ahead of the loop, where they would fill the scalar registers.

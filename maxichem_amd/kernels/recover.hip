@@ -4,7 +4,8 @@
 
 // Dense LU of W (NS x NS, row-major, LDS) in place: row k is exchanged with the row of the largest |entry| of column k
 // (ties to the lower row; a NaN is never chosen) only when |W_kk| < PIVTAU times it or is not finite. pr[i]: the
-// original row now in row i; dinv[i] = 1 / U_ii.
+// original row now in row i; dinv[i] = 1 / U_ii. The largest entry comes from a wave reduction of the lanes' own
+// largest, then its lowest row from a second one (the lowest row among the lanes holding the largest).
 __device__ void dense_lu(double* W, int* pr, double* dinv, int lane) {
   for (int i = lane; i < NS; i += WAVE) pr[i] = i;
   wave_sync();
@@ -18,16 +19,9 @@ __device__ void dense_lu(double* W, int* pr, double* dinv, int lane) {
         bi = i;
       }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ob = __shfl_xor(best, o, 64);
-      const int oi = __shfl_xor(bi, o, 64);
-      if (ob > best || (ob == best && oi < bi)) {
-        best = ob;
-        bi = oi;
-      }
-    }
-    const bool exchange = bi != NS && bi != k && !(fabs(W[k * NS + k]) >= PIVTAU * best);
+    const double top = wave_max(best);
+    bi = wave_min(best == top ? bi : NS);
+    const bool exchange = bi != NS && bi != k && !(fabs(W[k * NS + k]) >= PIVTAU * top);
     wave_sync();
     if (exchange) {
       for (int j = lane; j < NS; j += WAVE) {
@@ -65,28 +59,26 @@ __device__ void dense_lu(double* W, int* pr, double* dinv, int lane) {
   }
 }
 
-// y <- W^-1 y with dense_lu's factors; y is the vector in LDS (elimination order).
+// y <- W^-1 y with dense_lu's factors; y is the vector in LDS (elimination order). Unrolled with every lane's factor
+// entry loaded in bounds and its update kept by a select, so the chain from step to step is a readlane and an fma.
 __device__ void dense_solve(const double* W, const int* pr, const double* dinv, double* y, int lane) {
   double a[2];
 #pragma unroll
   for (int p = 0; p < 2; p++) a[p] = lane + WAVE * p < NS ? y[pr[lane + WAVE * p]] : 0.0;
-#pragma unroll 4
-  for (int k = 0; k < NS - 1; k++) {  // unrolled a little: the factor loads of later steps go out early
-    const double yk = readlane(k < WAVE ? a[0] : a[1], k & (WAVE - 1));
+  const int r0 = lane, r1 = lane + WAVE < NS ? lane + WAVE : NS - 1;  // rows in bounds (the second masked off)
 #pragma unroll
-    for (int p = 0; p < 2; p++) {
-      const int i = lane + WAVE * p;
-      if (i > k && i < NS) a[p] = fma(-W[i * NS + k], yk, a[p]);
-    }
+  for (int k = 0; k < NS - 1; k++) {
+    const double yk = readlane(k < WAVE ? a[0] : a[1], k & (WAVE - 1));
+    const double w0 = W[r0 * NS + k], w1 = W[r1 * NS + k];
+    a[0] = lane > k ? fma(-w0, yk, a[0]) : a[0];
+    if (NS > WAVE) a[1] = lane + WAVE > k && lane + WAVE < NS ? fma(-w1, yk, a[1]) : a[1];
   }
-#pragma unroll 4
+#pragma unroll
   for (int k = NS - 1; k > 0; k--) {
     const double zk = readlane(k < WAVE ? a[0] : a[1], k & (WAVE - 1)) * dinv[k];
-#pragma unroll
-    for (int p = 0; p < 2; p++) {
-      const int i = lane + WAVE * p;
-      if (i < k) a[p] = fma(-W[i * NS + k], zk, a[p]);
-    }
+    const double w0 = W[r0 * NS + k], w1 = W[r1 * NS + k];
+    a[0] = lane < k ? fma(-w0, zk, a[0]) : a[0];
+    if (NS > WAVE) a[1] = lane + WAVE < k ? fma(-w1, zk, a[1]) : a[1];
   }
   wave_sync();
 #pragma unroll

@@ -322,3 +322,82 @@ def test_guard_kernel_flags_drift_beyond_the_allowance(read_network, gpu):
     assert redo.tolist() == [int(r > 1.0) for r in DRIFTS] == want
     assert counts["drift"] == counts["recomputed"] == sum(r > 1.0 for r in DRIFTS)
 
+
+
+def kernel_sum(row, perm, k0, nt):
+    """The call kernel's screen sum of x (kernels/call.hip): lane l adds its lead entry x[perm[l]] (l < K0) and its
+    trailing entry x[perm[K0 + l]] (l < NT); the lanes are added in pairs, quads, half-rows and rows, then the four
+    rows as (row0 + row1) + (row2 + row3)."""
+    lanes = [(row[perm[l]] if l < k0 else 0.0) + (row[perm[k0 + l]] if l < nt else 0.0) for l in range(64)]
+    for width in (2, 4, 8, 16):
+        lanes = _fold(lanes, width)
+    r = [lanes[16 * q] for q in range(4)]
+    return (r[0] + r[1]) + (r[2] + r[3])
+
+
+def _fold(lanes, width):
+    """One DPP step of the kernel's reduction: every lane of each group of `width` gets the sum of its half and the
+    other half (a + b, which rounds as b + a)."""
+    out = list(lanes)
+    half = width // 2
+    for g in range(0, 64, width):
+        for i in range(g, g + width):
+            j = i ^ half if width <= 4 else (g + width - 1 - (i - g))  # pairs and quads by xor; halves and rows mirrored
+            out[i] = lanes[i] + lanes[j]
+    return out
+
+
+def on_the_tolerance(rows, perm, k0, nt, kernel_above):
+    """A row of the given states moved so that its sum is on the tolerance: above it in the kernel's order and not
+    above in index order (kernel_above), or the other way round."""
+    for row in rows:
+        base = np.asarray(row, np.float64)
+        j = int(np.argmax(base))
+        target = 1.0 + X_SUM_TOL - (sum(base.tolist()) - base[j])
+        for step in range(-400, 400):
+            r = base.copy()
+            r[j] = np.nextafter(target, np.inf) if step == 0 else target + step * np.spacing(target)
+            ks = kernel_sum(r.tolist(), perm, k0, nt) - 1.0
+            ns = sum(r.tolist()) - 1.0
+            if kernel_above and ks > X_SUM_TOL and not ns > X_SUM_TOL:
+                return r
+            if not kernel_above and not ks > X_SUM_TOL and ns > X_SUM_TOL:
+                return r
+    return None
+
+
+def test_sum_on_the_tolerance_is_judged_once(ncho):
+    # only the kernel's own sum of x is judged: a sum above X_SUM_TOL in its order and not in index order is
+    # refused; the opposite cell is accepted and stepped (the reference's test_guard.py:222-241)
+    net, solver = ncho
+    inp = workloads.config0(net.species, 0, cells=16, seed=3)
+    perm = solver.tab.perm.tolist()
+    above = on_the_tolerance(inp["x"], perm, solver.tab.k0, solver.tab.nt, True)
+    below = on_the_tolerance(inp["x"], perm, solver.tab.k0, solver.tab.nt, False)
+    assert above is not None and below is not None
+    T, P = t(inp["T"][:1]), t(inp["P"][:1])
+    dt, h = full(1, 50.0), full(1, 1e-10)
+    with pytest.raises(ValueError, match=r"1 cells of bad input: .*\(first: cell 0: sum\(x\) - 1 = "):
+        solver(t([above]), T, P, dt, h)
+    assert solver.fallback["bad_input"] == 1
+    r = solver(t([below]), T, P, dt, h)
+    assert bool(r.done.all()) and solver.fallback["bad_input"] == 0
+
+
+def test_close_releases_the_gpu_memory(read_network, gpu):
+    before = torch.cuda.memory_allocated()
+    solver = CudaSolver(read_network("synthetic_small"), "cuda")
+    held = torch.cuda.memory_allocated() - before
+    solver.close()
+    left = torch.cuda.memory_allocated() - before
+    print(f"holds {held} B, {left} B after close (the counts stay readable)")
+    assert left <= 4096 < held or left < held // 4
+
+
+def test_an_empty_batch_is_a_call(ncho):
+    net, solver = ncho
+    ns = len(net.species)
+    e = torch.empty(0, dtype=torch.float64, device="cuda")
+    r = solver.launch(torch.empty(0, ns, dtype=torch.float64, device="cuda"), e, e, e, e)
+    assert r.x.shape == (0, ns) and r.h.shape == (0,) and r.done.shape == (0,) and r.flag.shape == (0,)
+    assert int(solver.count[1:-1].abs().sum()) == 0

@@ -82,7 +82,7 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixe
     ints = {
         "WAVE": C.WAVE, "NS": tab.ns, "NR": tab.nr, "NRT": tab.nrt, "NJ": tab.nj, "K0": tab.k0, "NT": tab.nt,
         "NEL": tab.nel, "RCW": C.RATE_ROW, "RROUNDS": -(-tab.nr // C.WAVE), "NDV": tab.ndv, "NPART": tab.npart,
-        "CPW": cpw, "STREAM_BATCH": batch, "GUARD_THREADS": GUARD_THREADS, "CHUNK": CHUNK, "NCH": nch,
+        "CPW": cpw, "STREAM_BATCH": batch, "GUARD_THREADS": GUARD_THREADS, "CHUNK": CHUNK, "NCH": nch, "NLCH": tab.nlch,
         "SV_ROUNDS": tab.sv_words.shape[0], "DIAG_ROUNDS": tab.diag_words.shape[0],
         "LEAD_ROUNDS": tab.lead_words.shape[0], "REC_RETRIES": C.REC_RETRIES,
         "T_NEG": T_NEG, "T_DBL": T_DBL, "T_END": T_END, "T_DEST": T_DEST, "SV_NEG": SV_NEG, "SV_DBL": SV_DBL,

@@ -150,7 +150,8 @@ class Tables:
     ndv: int = 0
     diag_words: np.ndarray = field(default=None)  # uint32 [rounds, 64]
     lead_words: np.ndarray = field(default=None)  # uint32 [rounds, 64]
-    chunk_words: list = field(default_factory=list)  # per trailing chunk: uint32 [rounds, 64]
+    chunk_words: list = field(default_factory=list)  # per chunk (lead columns, then trailing): uint32 [rounds, 64]
+    nlch: int = 0  # the chunks of lead columns
     sv_words: np.ndarray = field(default=None)  # uint32 [rounds, 64]
     npart: int = 0
     row_pieces: np.ndarray = field(default=None)  # uint32 [ns]: first piece | count << 16
@@ -166,14 +167,13 @@ class Tables:
 
     # ---- the kernel's LDS layout (doubles per cell) ----
     def layout(self, mixed: bool = False, rates_lds: bool = True) -> dict[str, int]:
-        """Offsets (doubles) in a cell's LDS region: LEAD [K0][NS], A21 [NT][K0], n [NS + 1], dv [NDV + 1], then a
+        """Offsets (doubles) in a cell's LDS region: LEAD [K0][NS], n [NS + 1], dv [NDV + 1], then a
         union of what the rhs uses (v [NR + 1], pieces [NPART]) with what the Jacobian and the factorization use after
         it (the trailing diagonal [NT], staging [CHUNK][NT], column-major so that a lane's writes and reads of its row
         fall in distinct banks); G / RT [NS + 1] sits at its start while the rates are computed. mixed: the mixed-precision form's exchange vectors [2][NS + 1] follow; then the rate coefficients
         KF, KR [NR] (kept for the call) unless rates_lds is False (they are then in global scratch). Everything the
         Jacobian streams address lies before the optional parts, so the streams hold for every layout."""
-        a21 = self.k0 * self.ns
-        nvec = a21 + self.nt * self.k0
+        nvec = self.k0 * self.ns  # the trailing rows' lead columns (A21) are held in registers, one row per lane
         dv = nvec + self.ns + 1
         union = dv + self.ndv + 1
         v = union
@@ -189,7 +189,7 @@ class Tables:
         if rates_lds:
             size = kr + self.nr
         size += size & 1  # keep the next cell's region 16-byte aligned
-        return {"A21": a21, "NVEC": nvec, "KF": kf, "KR": kr, "DV": dv, "UNION": union, "V": v, "PART": part,
+        return {"NVEC": nvec, "KF": kf, "KR": kr, "DV": dv, "UNION": union, "V": v, "PART": part,
                 "DIAG": diag, "STAGE": stage, "XV": xv, "SIZE": size}
 
 
@@ -278,8 +278,12 @@ def build(net: VulcanNetwork, photo: bool = False) -> Tables:
         raise ValueError(f"a cell's LDS region of {reach} doubles exceeds the stream's destination field")
     zero = ndv  # dv[ndv] holds 0.0: the padding words' term
 
+    nlch = -(-k0 // CHUNK)  # chunks of the trailing rows' lead columns, before the trailing columns' chunks
+
     def dest(i: int, j: int) -> tuple[int, int]:
-        """(stream, LDS offset): stream -2 the diagonal, -1 the rest of the leading storage, else a trailing chunk.
+        """(stream, LDS offset): stream -2 the diagonal, -1 the rest of the lead rows, else a chunk of the trailing
+        rows: chunk c < nlch holds lead columns c CHUNK .. c CHUNK + CHUNK - 1, chunk nlch + c trailing columns
+        k0 + c CHUNK ... A chunk's entries land in the staging area, whence each lane takes its row into registers.
 
         The diagonal entries gather every derivative term of their species, so they are the long ones; they take
         a stream of their own, and the chunks' entries stay short."""
@@ -289,8 +293,8 @@ def build(net: VulcanNetwork, photo: bool = False) -> Tables:
             return -1, i * ns + j
         t = i - k0
         if j < k0:
-            return -1, lay["A21"] + t * k0 + j
-        c = (j - k0) // CHUNK
+            return j // CHUNK, lay["STAGE"] + (j % CHUNK) * nt + t
+        c = nlch + (j - k0) // CHUNK
         return c, lay["STAGE"] + ((j - k0) % CHUNK) * nt + t  # column-major: a column's rows are consecutive
 
     groups: dict[int, list[tuple[int, list[tuple[int, int]]]]] = {}
@@ -316,7 +320,8 @@ def build(net: VulcanNetwork, photo: bool = False) -> Tables:
 
     tab.diag_words = stream(groups.get(-2, []))
     tab.lead_words = stream(groups.get(-1, []))
-    tab.chunk_words = [stream(groups.get(c, [])) for c in range(-(-nt // CHUNK))]
+    tab.chunk_words = [stream(groups.get(c, [])) for c in range(nlch + -(-nt // CHUNK))]
+    tab.nlch = nlch
     # S v pieces: row i (elimination order) of S v from its reactions, cut into pieces of at most an even share
     rows = [[(r, int(S[perm[i], r])) for r in live if S[perm[i], r] != 0] for i in range(ns)]
     total = sum(len(t) for t in rows)

@@ -27,6 +27,7 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_wa
 #else
   double c[NT];
 #endif
+  LeadRow a21;  // the trailing row's lead columns of M0, then of L
   while (q < B) {
     const int cell = order ? order[q] : q;
     const double Tc = Ta[cell], dt = dta[cell], nd = Pa[cell] / (KBOLTZ * Tc);
@@ -65,21 +66,21 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_wa
       // F, w and -Jn at N; a try after a rejection is at the same N and computes the same values again
       double Fl = 0.0, Ft = 0.0, ul = 0.0, ut = 0.0, u0l = 0.0, u0t = 0.0;
       const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
-      jacobian(c, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
+      jacobian(c, a21, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
       add_diagonal(c, igh, ws, lane);
       double dl;
 #if MIXED
       float dtr;
-      factor_lead(c, dl, ws, lane);
+      factor_lead(c, a21, dl, ws, lane);
       factor_trailing32(c, dtr, ws, lane);
       const double wl = atol + rtol * fabs(Nl), wt = atol + rtol * fabs(Nt);  // the refinement's error weights
       bool lin_ok = true;  // every solve's refinement converged; else the try is rejected as a non-finite one
-#define MXC_SOLVE(NV, YL, YT) lin_ok = msolve<NV>(YL, YT, c, dl, dtr, igh, wl, wt, net, ws, lane) && lin_ok
+#define MXC_SOLVE(NV, YL, YT) lin_ok = msolve<NV>(YL, YT, c, a21, dl, dtr, igh, wl, wt, net, ws, lane) && lin_ok
 #else
       double dtr;
-      factor(c, dl, dtr, ws, lane);
+      factor(c, a21, dl, dtr, ws, lane);
       constexpr bool lin_ok = true;
-#define MXC_SOLVE(NV, YL, YT) solve<NV>(YL, YT, c, dl, dtr, ws, lane)
+#define MXC_SOLVE(NV, YL, YT) solve<NV>(YL, YT, c, a21, dl, dtr, ws, lane)
 #endif
       double y2l[2] = {Fl, ul}, y2t[2] = {Ft, ut};
       MXC_SOLVE(2, y2l, y2t);

@@ -2,9 +2,9 @@
 //
 // A cell's state vectors are held two values per lane: the lead part (elimination row `lane`, valid for lane < K0)
 // and the trailing part (row K0 + lane, valid for lane < NT). The trailing block of M0 = I / (gamma h) - Jn sits in
-// registers, lane t owning row K0 + t in c[NT]; the lead rows and the trailing rows' lead columns sit dense in the
-// cell's LDS region (LEAD [K0][NS], A21 [NT][K0]). The network header (generated per network) defines the sizes,
-// the LDS layout (L_*), the stream rounds and the method's settings.
+// registers, lane t owning row K0 + t in c[NT], and the same row's lead columns (A21) in a21[K0]; the lead rows sit
+// dense in the cell's LDS region (LEAD [K0][NS]). The network header (generated per network) defines the sizes, the
+// LDS layout (L_*), the stream rounds and the method's settings.
 
 typedef unsigned int u32;
 typedef unsigned long long u64;
@@ -24,6 +24,7 @@ constexpr int REFUSED_BAD_INPUT = 2;
 constexpr int REFUSED_HOT = 3;
 constexpr int BROADCAST_WINDOW = 32;  // v_readlane broadcasts of a pivot row in flight ahead of their updates
 constexpr int SMALL_LEAD = 8;  // at most this many lead rows: their factorization and solves are unrolled
+typedef double LeadRow[K0 > 0 ? K0 : 1];  // a trailing row's lead columns, one row per lane in registers
 
 // Orders the LDS accesses of the wavefront's lanes: the wave executes in lockstep and LDS serves one wave's accesses
 // in issue order, so only the compiler must not move loads and stores across this point.
@@ -337,13 +338,14 @@ __device__ __forceinline__ void jac_stream(const u32* __restrict__ words, int ro
   }
 }
 
-// -Jn into LEAD / A21 (LDS) and the trailing block c (registers), from the dv terms that rhs<true> left in DV; then
-// u = w = S v / nd - Jn N / n_tot, the constant-pressure part of the exact Jacobian (lead / trailing parts).
+// -Jn into LEAD (LDS), the trailing rows' lead columns a21 and the trailing block c (registers), from the dv terms
+// that rhs<true> left in DV; then u = w = S v / nd - Jn N / n_tot, the constant-pressure part of the exact Jacobian
+// (lead / trailing parts).
 template <typename T>
-__device__ __forceinline__ void jacobian(T (&c)[NT], double Nl, double Nt, double u0l, double u0t, double ntot,
-                                         double& ul, double& ut, const Net* net, double* ws, int lane) {
+__device__ __forceinline__ void jacobian(T (&c)[NT], LeadRow& a21, double Nl, double Nt, double u0l, double u0t,
+                                         double ntot, double& ul, double& ut, const Net* net, double* ws, int lane) {
   double* DV = ws + L_DV;
-  for (int s = lane; s < L_NVEC; s += WAVE) ws[s] = 0.0;  // LEAD and A21
+  for (int s = lane; s < L_NVEC; s += WAVE) ws[s] = 0.0;  // LEAD
   if (lane < NT) ws[L_DIAG + lane] = 0.0;                   // a species in no reaction has no diagonal term
   if (lane == 0) DV[NDV] = 0.0;                             // the padding words' term
   // the state N in the exchange vector (n's place: free once the dv terms are in DV)
@@ -354,8 +356,8 @@ __device__ __forceinline__ void jacobian(T (&c)[NT], double Nl, double Nt, doubl
   jac_stream(opaque(net)->diagw, DIAG_ROUNDS, DV, ws, lane);
   jac_stream(opaque(net)->leadw, LEAD_ROUNDS, DV, ws, lane);
   wave_sync();
-  // -Jn N in FP64, from the entries before the trailing block takes them in T: the lead rows and the lead columns
-  // of the trailing rows through the exchange vector, then the trailing columns chunk by chunk, N_j from lane j
+  // -Jn N in FP64, from the entries before the trailing block takes them in T: the lead rows through the exchange
+  // vector, then the trailing rows chunk by chunk (their lead columns, then the trailing ones), N_j from lane j
   double ml = 0.0, mt = 0.0;
   if constexpr (K0 > 0 && K0 <= SMALL_LEAD) {
     // few lead rows: each row times N as this lane's share (its own N entries), summed over the lanes; the trailing
@@ -366,8 +368,6 @@ __device__ __forceinline__ void jacobian(T (&c)[NT], double Nl, double Nt, doubl
       const double sum = wave_sum(part);
       if (lane == i) ml = sum;
     }
-#pragma unroll
-    for (int k = 0; k < K0; k++) mt = fma(lane < NT ? ws[L_A21 + lane * K0 + k] : 0.0, readlane(Nl, k), mt);
   } else if (K0 > 0) {
 #pragma unroll 1
     for (int j0 = 0; j0 < NS; j0 += STREAM_BATCH) {  // the lead rows, a batch of columns at a time
@@ -381,29 +381,37 @@ __device__ __forceinline__ void jacobian(T (&c)[NT], double Nl, double Nt, doubl
 #pragma unroll
       for (int i = 0; i < STREAM_BATCH; i++) ml = fma(a[i], nv[i], ml);
     }
-#pragma unroll 1
-    for (int k = 0; k < K0; k++) mt = fma(lane < NT ? ws[L_A21 + lane * K0 + k] : 0.0, xv[k], mt);
   }
   const double cd = lane < NT ? ws[L_DIAG + lane] : 0.0;
   double* stage = ws + L_STAGE;
+  // chunks 0 .. NLCH - 1 hold the lead columns (K0 > 0 means NT == WAVE: every lane has a trailing row), the
+  // others the trailing columns from t0
 #pragma unroll
   for (int ch = 0; ch < NCH; ch++) {
+    const int t0 = (ch - NLCH) * CHUNK;
     if (lane < NT) {
 #pragma unroll
       for (int j = 0; j < CHUNK; j++) stage[j * NT + lane] = 0.0;
       // the chunk streams hold no diagonal entry: a trailing row's own diagonal goes in by its lane
-      if (lane / CHUNK == ch) stage[(lane % CHUNK) * NT + lane] = cd;
+      if (ch >= NLCH && lane / CHUNK == ch - NLCH) stage[(lane % CHUNK) * NT + lane] = cd;
     }
     wave_sync();
     jac_stream(opaque(net)->chw + CH_OFF[ch] * WAVE, CH_ROUNDS[ch], DV, ws, lane);
     wave_sync();
 #pragma unroll
-    for (int j = 0; j < CHUNK; j++)
-      if (ch * CHUNK + j < NT) {
+    for (int j = 0; j < CHUNK; j++) {
+      if (ch < NLCH) {
+        if (ch * CHUNK + j < K0) {
+          const double v = stage[j * NT + lane];
+          a21[ch * CHUNK + j] = v;
+          mt = fma(v, readlane(Nl, ch * CHUNK + j), mt);
+        }
+      } else if (t0 + j < NT) {
         const double v = lane < NT ? stage[j * NT + lane] : 0.0;
-        c[ch * CHUNK + j] = (T)v;
-        mt = fma(v, readlane(Nt, ch * CHUNK + j), mt);
+        c[t0 + j] = (T)v;
+        mt = fma(v, readlane(Nt, t0 + j), mt);
       }
+    }
   }
   ul = u0l + ml / ntot;
   ut = lane < NT ? u0t + mt / ntot : 0.0;
@@ -435,17 +443,31 @@ __device__ __forceinline__ void schur_row(T (&c)[NT], double a, const double* ws
   }
 }
 
-// LU of the lead columns (right-looking, dense in LDS) and the Schur update of the trailing block (in T). dl: 1 / U_ii
-// of lead row lane.
+// The Schur update of the trailing block by the lead rows, one lead row at a time. This lane's multipliers go
+// through the staging area (free during the factorization) CHUNK at a time, so that the loop over the rows stays
+// rolled: unrolled, the compiler would load every row's broadcast values at once.
 template <typename T>
-__device__ __forceinline__ void factor_lead(T (&c)[NT], double& dl, double* ws, int lane) {
-  double* A21 = ws + L_A21;
+__device__ __forceinline__ void schur(T (&c)[NT], const LeadRow& a21, double* ws, int lane) {
+  double* stage = ws + L_STAGE;
+#pragma unroll
+  for (int k0 = 0; k0 < K0; k0 += CHUNK) {
+#pragma unroll
+    for (int j = 0; j < CHUNK; j++)
+      if (k0 + j < K0) stage[j * NT + lane] = a21[k0 + j];
+    wave_sync();
+#pragma unroll 1
+    for (int k = k0; k < (k0 + CHUNK < K0 ? k0 + CHUNK : K0); k++) schur_row(c, stage[(k - k0) * NT + lane], ws, k, lane);
+    wave_sync();
+  }
+}
+
+// LU of the lead columns (right-looking: the lead rows dense in LDS, this lane's trailing row in a21) and the Schur
+// update of the trailing block (in T). dl: 1 / U_ii of lead row lane.
+template <typename T>
+__device__ __forceinline__ void factor_lead(T (&c)[NT], LeadRow& a21, double& dl, double* ws, int lane) {
   dl = 1.0;
   if constexpr (K0 <= SMALL_LEAD) {
-    // few lead rows: the steps unrolled, and this lane's row of A21 in registers until the end
-    double a21[K0 > 0 ? K0 : 1];
-#pragma unroll
-    for (int k = 0; k < K0; k++) a21[k] = lane < NT ? A21[lane * K0 + k] : 0.0;
+    // few lead rows: the steps unrolled
 #pragma unroll
     for (int k = 0; k < K0; k++) {
       const double dk = 1.0 / ws[k * NS + k];
@@ -466,35 +488,26 @@ __device__ __forceinline__ void factor_lead(T (&c)[NT], double& dl, double* ws, 
       for (int j = k + 1; j < K0; j++) a21[j] = fma(-a21[k], ws[k * NS + j], a21[j]);
       wave_sync();
     }
-    if (lane < NT) {
-#pragma unroll
-      for (int k = 0; k < K0; k++) A21[lane * K0 + k] = a21[k];  // the solves read the multipliers from LDS
-    }
-    // the Schur update one lead row at a time (unrolled over k, the compiler would load every row at once)
-#pragma unroll 1
-    for (int k = 0; k < K0; k++) schur_row(c, lane < NT ? A21[lane * K0 + k] : 0.0, ws, k, lane);
-    wave_sync();
   } else {
-#pragma unroll 1
+    // the steps unrolled for the registers of a21, the lead rows' updates rolled
+#pragma unroll
     for (int k = 0; k < K0; k++) {
       const double dk = 1.0 / ws[k * NS + k];
       if (lane == k) dl = dk;
       if (lane > k && lane < K0) ws[lane * NS + k] *= dk;
-      if (lane < NT) A21[lane * K0 + k] *= dk;
+      a21[k] *= dk;
       wave_sync();
+#pragma unroll 1
       for (int i = k + 1; i < K0; i++) {
         const double l = ws[i * NS + k];
         for (int j = k + 1 + lane; j < NS; j += WAVE) ws[i * NS + j] = fma(-l, ws[k * NS + j], ws[i * NS + j]);
       }
-      if (lane < NT) {
-        const double a = A21[lane * K0 + k];
-        for (int j = k + 1; j < K0; j++) A21[lane * K0 + j] = fma(-a, ws[k * NS + j], A21[lane * K0 + j]);
-      }
+#pragma unroll
+      for (int j = k + 1; j < K0; j++) a21[j] = fma(-a21[k], ws[k * NS + j], a21[j]);
       wave_sync();
     }
-#pragma unroll 1
-    for (int k = 0; k < K0; k++) schur_row(c, lane < NT ? A21[lane * K0 + k] : 0.0, ws, k, lane);
   }
+  schur(c, a21, ws, lane);
 }
 
 // The trailing block's LU in registers (FP64). dt: 1 / U_ii of trailing row lane.
@@ -528,25 +541,24 @@ __device__ __forceinline__ void factor_trailing(double (&c)[NT], double& dt, dou
 
 // LU of the lead columns (right-looking, dense in LDS), the Schur update of the trailing block, then the trailing
 // block's LU in registers. dl: 1 / U_ii of lead row lane; dt: of trailing row lane.
-__device__ __forceinline__ void factor(double (&c)[NT], double& dl, double& dt, double* ws, int lane) {
-  factor_lead(c, dl, ws, lane);
+__device__ __forceinline__ void factor(double (&c)[NT], LeadRow& a21, double& dl, double& dt, double* ws, int lane) {
+  factor_lead(c, a21, dl, ws, lane);
   factor_trailing(c, dt, ws, lane);
 }
 
 // y <- M0^-1 y for NV vectors (lead / trailing parts), with factor's factors, in three parts: the lead columns of L,
 // the trailing block, the lead rows of U.
 template <int NV>
-__device__ __forceinline__ void solve_lead_forward(double (&yl)[NV], double (&yt)[NV], double* ws, int lane) {
-  const double* A21 = ws + L_A21;
-#pragma unroll(K0 <= SMALL_LEAD ? SMALL_LEAD : 1)
-  for (int k = 0; k < K0; k++) {  // unit L, lead columns
+__device__ __forceinline__ void solve_lead_forward(double (&yl)[NV], double (&yt)[NV], const LeadRow& a21,
+                                                   const double* ws, int lane) {
+#pragma unroll
+  for (int k = 0; k < K0; k++) {  // unit L, lead columns (the trailing rows' entries in a21)
     const double lk = (lane > k && lane < K0) ? ws[lane * NS + k] : 0.0;
-    const double tk = lane < NT ? A21[lane * K0 + k] : 0.0;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       const double yk = readlane(yl[v], k);
       yl[v] = fma(-lk, yk, yl[v]);
-      yt[v] = fma(-tk, yk, yt[v]);
+      yt[v] = fma(-a21[k], yk, yt[v]);
     }
   }
 }
@@ -643,9 +655,9 @@ __device__ __forceinline__ void solve_lead_backward(double (&yl)[NV], const doub
 }
 
 template <int NV>
-__device__ __forceinline__ void solve(double (&yl)[NV], double (&yt)[NV], const double (&c)[NT], double dl, double dt,
-                                      double* ws, int lane) {
-  solve_lead_forward(yl, yt, ws, lane);
+__device__ __forceinline__ void solve(double (&yl)[NV], double (&yt)[NV], const double (&c)[NT], const LeadRow& a21,
+                                      double dl, double dt, double* ws, int lane) {
+  solve_lead_forward(yl, yt, a21, ws, lane);
   solve_trailing(yt, c, dt, lane);
   solve_lead_backward(yl, yt, dl, ws, lane);
 }

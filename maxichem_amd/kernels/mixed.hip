@@ -149,12 +149,13 @@ __device__ __forceinline__ void m0_times(const double (&xl)[NV], const double (&
 // sweep's correction fell below REFINE_TOL weights: where M0 is too ill-conditioned for FP32 (large steps of stiff
 // chemistry) the refinement stalls or diverges, and the try is then not to be trusted.
 template <int NV>
-__device__ __forceinline__ bool msolve(double (&yl)[NV], double (&yt)[NV], const float (&c)[NT], double dl, float dt,
-                                       double igh, double wl, double wt, const Net* net, double* ws, int lane) {
+__device__ __forceinline__ bool msolve(double (&yl)[NV], double (&yt)[NV], const float (&c)[NT], const LeadRow& a21,
+                                       double dl, float dt, double igh, double wl, double wt, const Net* net,
+                                       double* ws, int lane) {
   double bl[NV], bt[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) bl[v] = yl[v], bt[v] = yt[v];
-  solve_lead_forward(yl, yt, ws, lane);
+  solve_lead_forward(yl, yt, a21, ws, lane);
   solve_trailing32(yt, c, dt, lane);
   solve_lead_backward(yl, yt, dl, ws, lane);
 #pragma unroll 1
@@ -163,7 +164,7 @@ __device__ __forceinline__ bool msolve(double (&yl)[NV], double (&yt)[NV], const
     m0_times(yl, yt, rl, rt, igh, net, ws, lane);
 #pragma unroll
     for (int v = 0; v < NV; v++) rl[v] = bl[v] - rl[v], rt[v] = bt[v] - rt[v];
-    solve_lead_forward(rl, rt, ws, lane);
+    solve_lead_forward(rl, rt, a21, ws, lane);
     solve_trailing32(rt, c, dt, lane);
     solve_lead_backward(rl, rt, dl, ws, lane);
     double worst = 0.0;

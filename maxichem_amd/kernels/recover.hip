@@ -123,6 +123,7 @@ extern "C" __global__ void __launch_bounds__(WAVE)
   double* xo = yv + NS;
   int* pr = (int*)(xo + NS);
   double c[NT];
+  LeadRow a21;
   int q = (int)blockIdx.x;
   while (q < nfl) {
     const int cell = list[q];
@@ -144,7 +145,7 @@ extern "C" __global__ void __launch_bounds__(WAVE)
         const double ha = fmin(s.h, dt - s.t), igh = 1.0 / (GAMMA * ha), ih = 1.0 / ha;
         double Fl = 0.0, Ft = 0.0, ul = 0.0, ut = 0.0, u0l = 0.0, u0t = 0.0;
         const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
-        jacobian(c, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
+        jacobian(c, a21, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
         // W = I / (gamma h) - Jn - w 1^T, dense (recover.cuh:124-130)
         to_lds(yv, ul, ut, lane);
         for (int e = lane; e < K0 * NS; e += WAVE) {
@@ -153,7 +154,8 @@ extern "C" __global__ void __launch_bounds__(WAVE)
         }
         if (lane < NT) {
           double* row = W + (K0 + lane) * NS;
-          for (int k = 0; k < K0; k++) row[k] = ws[L_A21 + lane * K0 + k] - ut;
+#pragma unroll
+          for (int k = 0; k < K0; k++) row[k] = a21[k] - ut;
 #pragma unroll
           for (int j = 0; j < NT; j++) row[K0 + j] = (j == lane ? c[j] + igh : c[j]) - ut;
         }

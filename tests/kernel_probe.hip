@@ -16,12 +16,13 @@ extern "C" __global__ void __launch_bounds__(WAVE)
   double* o = out + 8;
   const double nd = Pc / (KBOLTZ * Tc);
   double c[NT];
+  LeadRow a21;
   cell_rates(ws, ws + L_KF, Tc, nd, net, Jc, lane);
   const int* perm = net->perm;
   double Nl = lane < K0 ? xr[perm[lane]] : 0.0, Nt = lane < NT ? xr[perm[K0 + lane]] : 0.0;
   double Fl = 0.0, Ft = 0.0, ul = 0.0, ut = 0.0, u0l = 0.0, u0t = 0.0;
   const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, ws + L_KF, lane);
-  jacobian(c, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
+  jacobian(c, a21, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
   auto put = [&](double* dst, double vl, double vt) {
     if (lane < K0) dst[lane] = vl;
     if (lane < NT) dst[K0 + lane] = vt;
@@ -31,7 +32,8 @@ extern "C" __global__ void __launch_bounds__(WAVE)
   double* jn = o + 2 * NS;  // -Jn, dense
   for (int e = lane; e < K0 * NS; e += WAVE) jn[e] = ws[e];
   if (lane < NT) {
-    for (int k = 0; k < K0; k++) jn[(K0 + lane) * NS + k] = ws[L_A21 + lane * K0 + k];
+#pragma unroll
+    for (int k = 0; k < K0; k++) jn[(K0 + lane) * NS + k] = a21[k];
     for (int j = 0; j < NT; j++) jn[(K0 + lane) * NS + K0 + j] = c[j];
   }
   double* lu = jn + NS * NS;  // the factors of M0 (unit L below the diagonal, U on and above), then 1 / U_ii
@@ -39,16 +41,17 @@ extern "C" __global__ void __launch_bounds__(WAVE)
   const double igh = 1.0 / (GAMMA * ha), ih = 1.0 / ha;
   add_diagonal(c, igh, ws, lane);
   double dl, dtr;
-  factor(c, dl, dtr, ws, lane);
+  factor(c, a21, dl, dtr, ws, lane);
   wave_sync();
   for (int e = lane; e < K0 * NS; e += WAVE) lu[e] = ws[e];
   if (lane < NT) {
-    for (int k = 0; k < K0; k++) lu[(K0 + lane) * NS + k] = ws[L_A21 + lane * K0 + k];
+#pragma unroll
+    for (int k = 0; k < K0; k++) lu[(K0 + lane) * NS + k] = a21[k];
     for (int j = 0; j < NT; j++) lu[(K0 + lane) * NS + K0 + j] = c[j];
   }
   put(lu + NS * NS, dl, dtr);
   double y2l[2] = {Fl, ul}, y2t[2] = {Ft, ut};
-  solve<2>(y2l, y2t, c, dl, dtr, ws, lane);
+  solve<2>(y2l, y2t, c, a21, dl, dtr, ws, lane);
   put(v, y2l[0], y2t[0]);
   const double Ul = y2l[1], Ut = y2t[1];
   const double c0 = 1.0 / (1.0 - vsum(Ul, Ut, lane));
@@ -56,7 +59,7 @@ extern "C" __global__ void __launch_bounds__(WAVE)
   const double K1l = y2l[0] + sk * Ul, K1t = y2t[0] + sk * Ut;
   put(v + NS, K1l, K1t);
   double y1l[1] = {fma(RODAS_C21 * ih, K1l, Fl)}, y1t[1] = {fma(RODAS_C21 * ih, K1t, Ft)};
-  solve<1>(y1l, y1t, c, dl, dtr, ws, lane);
+  solve<1>(y1l, y1t, c, a21, dl, dtr, ws, lane);
   sk = vsum(y1l[0], y1t[0], lane) * c0;
   const double K2l = y1l[0] + sk * Ul, K2t = y1t[0] + sk * Ut;
   put(v + 2 * NS, K2l, K2t);
@@ -64,7 +67,7 @@ extern "C" __global__ void __launch_bounds__(WAVE)
   rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, ws + L_KF, lane);
   y1l[0] = fma(ih, K1l - K2l, Fl);
   y1t[0] = fma(ih, K1t - K2t, Ft);
-  solve<1>(y1l, y1t, c, dl, dtr, ws, lane);
+  solve<1>(y1l, y1t, c, a21, dl, dtr, ws, lane);
   sk = vsum(y1l[0], y1t[0], lane) * c0;
   const double K3l = y1l[0] + sk * Ul, K3t = y1t[0] + sk * Ut;
   put(v + 3 * NS, K3l, K3t);
@@ -73,7 +76,7 @@ extern "C" __global__ void __launch_bounds__(WAVE)
   rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, ws + L_KF, lane);
   y1l[0] = fma(ih, fma(-RODAS_C43, K3l, K1l - K2l), Fl);
   y1t[0] = fma(ih, fma(-RODAS_C43, K3t, K1t - K2t), Ft);
-  solve<1>(y1l, y1t, c, dl, dtr, ws, lane);
+  solve<1>(y1l, y1t, c, a21, dl, dtr, ws, lane);
   sk = vsum(y1l[0], y1t[0], lane) * c0;
   put(v + 4 * NS, y1l[0] + sk * Ul, y1t[0] + sk * Ut);
 }

@@ -18,15 +18,15 @@ def test_layout_regions_that_live_together_do_not_overlap(read_network):
         tab = _tables.build(net, photo)
         L = tab.layout()
         regions = {
-            "lead": (0, L["A21"]), "a21": (L["A21"], L["NVEC"]), "n": (L["NVEC"], L["NVEC"] + tab.ns + 1),
+            "lead": (0, L["NVEC"]), "n": (L["NVEC"], L["NVEC"] + tab.ns + 1),
             "dv": (L["DV"], L["DV"] + tab.ndv + 1), "v": (L["V"], L["V"] + tab.nr + 1),
             "kf": (L["KF"], L["KF"] + tab.nr), "kr": (L["KR"], L["KR"] + tab.nr),
             "part": (L["PART"], L["PART"] + tab.npart), "diag": (L["DIAG"], L["DIAG"] + tab.nt),
             "stage": (L["STAGE"], L["STAGE"] + tab.nt * CHUNK),
         }  # fmt: skip
         # live at once: the rhs with the Jacobian's dv; the Jacobian's assembly; the factorization
-        groups = [("lead", "a21", "n", "kf", "kr", "dv", "v", "part"),
-                  ("lead", "a21", "n", "kf", "kr", "dv", "diag", "stage")]
+        groups = [("lead", "n", "kf", "kr", "dv", "v", "part"),
+                  ("lead", "n", "kf", "kr", "dv", "diag", "stage")]
         for g in groups:
             spans = sorted(regions[k] for k in g)
             for (a0, a1), (b0, b1) in zip(spans, spans[1:]):
@@ -82,10 +82,45 @@ def test_every_layout_keeps_the_offsets_the_streams_address(read_network):
     for net in (read_network("synthetic"), standin.load("NCHO")):
         tab = _tables.build(net, True)
         base = tab.layout()
-        fixed = ("A21", "NVEC", "DV", "UNION", "V", "PART", "DIAG", "STAGE")
+        fixed = ("NVEC", "DV", "UNION", "V", "PART", "DIAG", "STAGE")
         for mixed in (False, True):
             for rates in (False, True):
                 lay = tab.layout(mixed, rates)
                 assert all(lay[k] == base[k] for k in fixed)
                 assert lay["SIZE"] >= max(lay["XV"] + (2 * (tab.ns + 1) if mixed else 0),
                                           (lay["KR"] + tab.nr) if rates else 0)  # fmt: skip
+
+
+def test_chunks_stage_the_trailing_rows_by_column(read_network):
+    # the trailing rows' entries reach the staging area chunk by chunk: the first nlch chunks hold the lead columns
+    # (kept in registers, a21), the others the trailing columns; each word's destination names its row and column
+    for net, photo in _nets(read_network):
+        tab = _tables.build(net, photo)
+        L = tab.layout()
+        S = np.hstack([net.S, net.Sp]) if photo else np.asarray(net.S)
+        pos = np.empty(tab.ns, np.int64)
+        pos[tab.perm] = np.arange(tab.ns)
+        assert tab.nlch == -(-tab.k0 // CHUNK) and len(tab.chunk_words) == tab.nlch + -(-tab.nt // CHUNK)
+        got = set()
+        for c, words in enumerate(tab.chunk_words):
+            for w in words.ravel():
+                if (w >> T_END) & 1:
+                    off = (int(w) >> T_DEST) - L["STAGE"]
+                    assert 0 <= off < CHUNK * tab.nt
+                    slot, t = divmod(off, tab.nt)
+                    col = c * CHUNK + slot if c < tab.nlch else tab.k0 + (c - tab.nlch) * CHUNK + slot
+                    assert (col < tab.k0) == (c < tab.nlch) and col != tab.k0 + t
+                    got.add((tab.k0 + t, col))
+        # every off-diagonal entry of a trailing row that a reaction couples, in elimination order
+        want = set()
+        for r in range(S.shape[1]):
+            rows = np.flatnonzero(S[:, r])
+            if rows.size == 0:
+                continue
+            for q in (list(net.reac) + (list(net.photo) if photo else []))[r]["re"] + (
+                    (list(net.reac) + (list(net.photo) if photo else []))[r]["pr"] if r < len(net.reac) else []):
+                for p_ in rows:
+                    i, j = int(pos[p_]), int(pos[q])
+                    if i >= tab.k0 and i != j:
+                        want.add((i, j))
+        assert got == want

@@ -42,6 +42,7 @@ OPTIONS = (
     "-packed-fp32-ops",
 )
 GUARD_THREADS = 256
+REC_WAVES = 4  # wavefronts of a recovery workgroup: the first steps the cell, all of them share its dense LU
 KERNEL_NAMES = ("mxc_call", "mxc_guard", "mxc_recover")
 # Each kernel's parameters in order (kernels/*.hip); an export lists them and the C++ loader refuses another list, as
 # the reference's KERNEL_ARGUMENTS (maxichem/_constants.py:148-164) does for its kernels.
@@ -82,7 +83,8 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixe
     ints = {
         "WAVE": C.WAVE, "NS": tab.ns, "NR": tab.nr, "NRT": tab.nrt, "NJ": tab.nj, "K0": tab.k0, "NT": tab.nt,
         "NEL": tab.nel, "RCW": C.RATE_ROW, "RROUNDS": -(-tab.nr // C.WAVE), "NDV": tab.ndv, "NPART": tab.npart,
-        "CPW": cpw, "STREAM_BATCH": batch, "GUARD_THREADS": GUARD_THREADS, "CHUNK": CHUNK, "NCH": nch, "NLCH": tab.nlch,
+        "CPW": cpw, "STREAM_BATCH": batch, "GUARD_THREADS": GUARD_THREADS, "REC_WAVES": REC_WAVES, "CHUNK": CHUNK,
+        "NCH": nch, "NLCH": tab.nlch,
         "SV_ROUNDS": tab.sv_words.shape[0], "DIAG_ROUNDS": tab.diag_words.shape[0],
         "LEAD_ROUNDS": tab.lead_words.shape[0], "REC_RETRIES": C.REC_RETRIES,
         "T_NEG": T_NEG, "T_DBL": T_DBL, "T_END": T_END, "T_DEST": T_DEST, "SV_NEG": SV_NEG, "SV_DBL": SV_DBL,
@@ -115,9 +117,10 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixe
 
 
 def lds_bytes(tab: Tables, cpw: int, mixed: bool = False, rates_global: bool = False) -> tuple[int, int]:
-    """Dynamic LDS of a call workgroup (cpw cells) and of a recovery workgroup (one cell and its dense W)."""
+    """Dynamic LDS of a call workgroup (cpw cells) and of a recovery workgroup (one cell, its dense W and its
+    factors' row order, and the word by which its first wavefront calls the others to a factorization)."""
     size = tab.layout(mixed, not rates_global)["SIZE"]
-    return 8 * size * cpw, 8 * (size + tab.ns * tab.ns + 3 * tab.ns) + 4 * tab.ns
+    return 8 * size * cpw, 8 * (size + tab.ns * tab.ns + 3 * tab.ns) + 4 * tab.ns + 4
 
 
 def host_tables(tab: Tables) -> dict[str, np.ndarray]:

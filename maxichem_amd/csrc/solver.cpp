@@ -26,7 +26,7 @@ using mxc_detail::fail;
 
 namespace {
 
-constexpr int kExportFormat = 1;
+constexpr int kExportFormat = 2;  // 2: the recovery workgroup's size (rec_threads)
 constexpr int kWave = 64;  // lanes of a wavefront: one wavefront steps one cell
 // The kernels' Net struct, one device pointer per table in this order (kernels/common.hip).
 const char* const kNetFields[] = {"perm", "rc", "rf", "nasa", "rxe", "svw", "rowp", "diagw", "leadw", "chw", "atoms"};
@@ -86,6 +86,7 @@ struct mxc_solver {
   hipModule_t module = nullptr;
   hipFunction_t call = nullptr, guard = nullptr, recover = nullptr;
   int ns = 0, nj = 0, nr = 0, nel = 0, cpw = 1, lds_call = 0, lds_rec = 0, rates_global = 0, guard_threads = 0;
+  int rec_threads = 0;  // the recovery workgroup: wavefront 0 steps the cell, all of them share its dense LU
   int rec_steps = 0, max_steps = 0, max_blocks = 0, rec_blocks = 0;
   int bad_slot = -1, cold_slot = -1, hot_slot = -1;
   double rtol = 0.0, atol = 0.0, t_min = 0.0, t_max = 0.0, x_sum_tol = 0.0;
@@ -171,11 +172,13 @@ int read_export(const std::string& folder, mxc_solver* s, std::string& code, std
   if (integer("ns", s->ns) || integer("nj", s->nj) || integer("nr", s->nr) || integer("nel", s->nel) ||
       integer("wave", wave) || integer("cpw", s->cpw) || integer("lds_call", s->lds_call) ||
       integer("lds_rec", s->lds_rec) || integer("rates_global", s->rates_global) ||
-      integer("guard_threads", s->guard_threads) || integer("rec_steps", s->rec_steps) ||
+      integer("guard_threads", s->guard_threads) || integer("rec_threads", s->rec_threads) ||
+      integer("rec_steps", s->rec_steps) ||
       integer("max_steps", s->max_steps) || real("atol", s->atol) || real("rtol", s->rtol) ||
       real("t_min", s->t_min) || real("t_max", s->t_max) || real("x_sum_tol", s->x_sum_tol))
     return 1;
   if (wave != kWave || s->ns < 1 || s->nj < 0 || s->nr < 1 || s->nel < 1 || s->cpw < 1 || s->guard_threads < 1 ||
+      s->rec_threads < kWave || s->rec_threads % kWave ||
       s->rec_steps < 1 || s->lds_call < 0 || s->lds_rec < 0)
     return fail("mxc_solver_open: meta.txt: sizes out of range");
   s->species = meta["species"];
@@ -375,7 +378,8 @@ int mxc_solver_open(const char* folder, int device, mxc_solver** solver) {
       check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "compute units") ||
       check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ_call, s->call, kWave * s->cpw, s->lds_call),
             "occupancy") ||
-      check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ_rec, s->recover, kWave, s->lds_rec), "occupancy");
+      check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ_rec, s->recover, s->rec_threads, s->lds_rec),
+            "occupancy");
   if (bad) {
     release(s);
     return 1;
@@ -500,7 +504,7 @@ int mxc_solver_call(mxc_solver* s, void* stream_, int cells, const double* x, co
   if (launch(1, s->guard, guard_grid, s->guard_threads, 0)) return 1;
   v["max_steps"] = &rec_steps;
   const unsigned rec_grid = static_cast<unsigned>(std::max(1, std::min(cells, s->rec_blocks)));
-  if (launch(2, s->recover, rec_grid, kWave, s->lds_rec)) return 1;
+  if (launch(2, s->recover, rec_grid, s->rec_threads, s->lds_rec)) return 1;
   if (check(hipMemcpyAsync(s->last.data(), s->count, slots * sizeof(int), hipMemcpyDeviceToHost, stream),
             "copy counts") ||
       check(hipStreamSynchronize(stream), "hipStreamSynchronize"))  // the call's one host sync

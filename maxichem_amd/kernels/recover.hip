@@ -2,13 +2,21 @@
 // call kernel's rates, rhs, Jacobian, stages and step control, but the dense W = I / (gamma h) - Jn - w 1^T factored
 // by LU with threshold partial pivoting (the reference's recover kernel, maxichem/kernels/recover.cuh:3-310).
 
-// Dense LU of W (NS x NS, row-major, LDS) in place: row k is exchanged with the row of the largest |entry| of column k
-// (ties to the lower row; a NaN is never chosen) only when |W_kk| < PIVTAU times it or is not finite. pr[i]: the
-// original row now in row i; dinv[i] = 1 / U_ii. The largest entry comes from a wave reduction of the lanes' own
-// largest, then its lowest row from a second one (the lowest row among the lanes holding the largest).
-__device__ void dense_lu(double* W, int* pr, double* dinv, int lane) {
-  for (int i = lane; i < NS; i += WAVE) pr[i] = i;
-  wave_sync();
+// Dense LU of W (NS x NS, row-major, LDS) in place, shared by the workgroup's REC_WAVES wavefronts (wv: this one's
+// index): row k is exchanged with the row of the largest |entry| of column k (ties to the lower row; a NaN is never
+// chosen) only when |W_kk| < PIVTAU times it or is not finite. pr[i]: the original row now in row i; dinv[i] =
+// 1 / U_ii. Every wavefront finds the pivot itself from the same column, so all take the same branches and meet at
+// the same barriers: the largest entry from a wave reduction of the lanes' own largest, then its lowest row from a
+// second one. Lane l updates rows k + 1 + l (+ WAVE); wavefront wv the blocks of REC_BLOCK columns b = wv, wv +
+// REC_WAVES, ... from column k + 1, so every entry takes the same operations in the same order as with one
+// wavefront.
+constexpr int REC_BLOCK = 8;
+constexpr int REC_ROWS = (NS + WAVE - 2) / WAVE;  // rows below a pivot per lane
+
+__device__ void dense_lu(double* W, int* pr, double* dinv, int lane, int wv) {
+  if (wv == 0)
+    for (int i = lane; i < NS; i += WAVE) pr[i] = i;
+  __syncthreads();
   for (int k = 0; k < NS; k++) {
     double best = -1.0;
     int bi = NS;
@@ -22,40 +30,51 @@ __device__ void dense_lu(double* W, int* pr, double* dinv, int lane) {
     const double top = wave_max(best);
     bi = wave_min(best == top ? bi : NS);
     const bool exchange = bi != NS && bi != k && !(fabs(W[k * NS + k]) >= PIVTAU * top);
-    wave_sync();
     if (exchange) {
-      for (int j = lane; j < NS; j += WAVE) {
+      __syncthreads();  // every wavefront has read column k
+      for (int j = lane + WAVE * wv; j < NS; j += WAVE * REC_WAVES) {
         const double a = W[k * NS + j];
         W[k * NS + j] = W[bi * NS + j];
         W[bi * NS + j] = a;
       }
-      if (lane == 0) {
+      if (wv == 0 && lane == 0) {
         const int t = pr[k];
         pr[k] = pr[bi];
         pr[bi] = t;
       }
+      __syncthreads();
     }
-    wave_sync();
     const double dk = 1.0 / W[k * NS + k];
-    if (lane == 0) dinv[k] = dk;
-    for (int i = k + 1 + lane; i < NS; i += WAVE) {
-      const double l = W[i * NS + k] * dk;
-      W[i * NS + k] = l;
-      // eight columns at a time: their loads go out together instead of one LDS round trip per column
-      int j = k + 1;
-      for (; j + 8 <= NS; j += 8) {
-        double u[8], w[8];
+    if (wv == 0 && lane == 0) dinv[k] = dk;
+    double l[REC_ROWS];
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-          u[q] = W[k * NS + j + q];
-          w[q] = W[i * NS + j + q];
-        }
-#pragma unroll
-        for (int q = 0; q < 8; q++) W[i * NS + j + q] = fma(-l, u[q], w[q]);
-      }
-      for (; j < NS; j++) W[i * NS + j] = fma(-l, W[k * NS + j], W[i * NS + j]);
+    for (int p = 0; p < REC_ROWS; p++) {
+      const int i = k + 1 + lane + WAVE * p;
+      l[p] = i < NS ? W[i * NS + k] * dk : 0.0;
     }
-    wave_sync();
+    __syncthreads();  // every wavefront has read column k before the first writes its multipliers
+#pragma unroll
+    for (int p = 0; p < REC_ROWS; p++) {
+      const int i = k + 1 + lane + WAVE * p;
+      if (i < NS) {
+        if (wv == 0) W[i * NS + k] = l[p];
+        for (int j0 = k + 1 + REC_BLOCK * wv; j0 < NS; j0 += REC_BLOCK * REC_WAVES) {
+          if (j0 + REC_BLOCK <= NS) {  // a whole block: its loads go out together
+            double u[REC_BLOCK], w[REC_BLOCK];
+#pragma unroll
+            for (int q = 0; q < REC_BLOCK; q++) {
+              u[q] = W[k * NS + j0 + q];
+              w[q] = W[i * NS + j0 + q];
+            }
+#pragma unroll
+            for (int q = 0; q < REC_BLOCK; q++) W[i * NS + j0 + q] = fma(-l[p], u[q], w[q]);
+          } else {
+            for (int j = j0; j < NS; j++) W[i * NS + j] = fma(-l[p], W[k * NS + j], W[i * NS + j]);
+          }
+        }
+      }
+    }
+    __syncthreads();  // the updates are in before the next column is searched
   }
 }
 
@@ -100,7 +119,10 @@ __device__ __forceinline__ void from_lds(const double* y, double& vl, double& vt
   wave_sync();
 }
 
-extern "C" __global__ void __launch_bounds__(WAVE)
+// A workgroup of REC_WAVES wavefronts per listed cell: wavefront 0 steps the cell; the others wait at a workgroup
+// barrier for each of its dense factorizations (cmd 1: factor W, 0: the workgroup's cells are done), so every
+// wavefront passes the same barriers.
+extern "C" __global__ void __launch_bounds__(WAVE* REC_WAVES)
     mxc_recover(const Net* __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
                 const double* __restrict__ Pa, const double* __restrict__ dta, const double* __restrict__ hin,
                 double* __restrict__ xout, double* __restrict__ hio, int* __restrict__ doneo,
@@ -110,7 +132,7 @@ extern "C" __global__ void __launch_bounds__(WAVE)
   const int nfl = count[COUNT_RECOMPUTED];
   if ((int)blockIdx.x >= nfl) return;
   extern __shared__ double lds[];
-  const int lane = (int)threadIdx.x;
+  const int lane = (int)(threadIdx.x & (WAVE - 1)), wv = (int)(threadIdx.x / WAVE);
   double* ws = lds;
 #if RATES_GLOBAL
   double* rk = rscratch + (size_t)blockIdx.x * (2 * NR);  // this workgroup's slot
@@ -122,6 +144,14 @@ extern "C" __global__ void __launch_bounds__(WAVE)
   double* yv = dinv + NS;
   double* xo = yv + NS;
   int* pr = (int*)(xo + NS);
+  volatile int* cmd = pr + NS;
+  if (wv != 0) {
+    for (;;) {
+      __syncthreads();
+      if (*cmd == 0) return;
+      dense_lu(W, pr, dinv, lane, wv);
+    }
+  }
   double c[NT];
   LeadRow a21;
   int q = (int)blockIdx.x;
@@ -159,8 +189,9 @@ extern "C" __global__ void __launch_bounds__(WAVE)
 #pragma unroll
           for (int j = 0; j < NT; j++) row[K0 + j] = (j == lane ? c[j] + igh : c[j]) - ut;
         }
-        wave_sync();
-        dense_lu(W, pr, dinv, lane);
+        if (lane == 0) *cmd = 1;
+        __syncthreads();
+        dense_lu(W, pr, dinv, lane, 0);
         double K1l, K1t, K2l, K2t, K3l, K3t, K4l, K4t;
         to_lds(yv, Fl, Ft, lane);
         dense_solve(W, pr, dinv, yv, lane);
@@ -221,4 +252,6 @@ extern "C" __global__ void __launch_bounds__(WAVE)
       break;
     }
   }
+  if (lane == 0) *cmd = 0;
+  __syncthreads();  // the other wavefronts leave
 }

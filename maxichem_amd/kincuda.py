@@ -121,7 +121,7 @@ class CudaSolver(Solver):
         self._recover = self._module.function("mxc_recover")
         cus = info["compute_units"]
         self.max_blocks = cus * max(1, _hip.occupancy(self._call, WAVE * self.cpw, self.lds_call))
-        self.rec_blocks = cus * max(1, _hip.occupancy(self._recover, WAVE, self.lds_rec))
+        self.rec_blocks = cus * max(1, _hip.occupancy(self._recover, WAVE * _kernels.REC_WAVES, self.lds_rec))
         # each call wavefront's and each recovery workgroup's slot of rate coefficients, when they are not in LDS
         slots = max(self.max_blocks * self.cpw, self.rec_blocks)
         self._rscratch = (torch.empty(slots * 2 * tab.nr, dtype=torch.float64, device=self.dev)
@@ -213,7 +213,8 @@ class CudaSolver(Solver):
         rec_args = [net, p(x), p(T), p(P), p(dt), p(h), p(xo), p(ho), p(done), p(nok), p(redo), p(lst), p(self.count),
                     p(self.tries), p(jx), ctypes.c_double(rtol), ctypes.c_double(atol),
                     ctypes.c_int(REC_STEPS * max_steps), p(self._rscratch)]  # fmt: skip
-        _hip.launch(self._recover, max(1, min(B, self.rec_blocks)), WAVE, self.lds_rec, stream, rec_args)
+        _hip.launch(self._recover, max(1, min(B, self.rec_blocks)), WAVE * _kernels.REC_WAVES, self.lds_rec, stream,
+                    rec_args)
 
     def export(self, folder) -> None:
         """Write what a caller without Python needs to run this solver's kernels (maxichem/kincuda.py:363-373): the

@@ -84,7 +84,8 @@ def open_error(folder: Path) -> str:
 
 
 EDITS = {  # meta.txt edits mxc_solver_open refuses before any device work, and the message each gives
-    "format": ("export_format 1", "export_format 2", "export format 2, this library reads 1"),
+    "format": ("export_format 2", "export_format 3", "export format 3, this library reads 2"),
+    "no recovery workgroup": ("rec_threads 256", "rec_threads 96", "sizes out of range"),
     "unknown": ("args mxc_recover net", "args mxc_recover net2", "kernel mxc_recover takes argument net2, which"),
     "repeated": ("args mxc_guard net x0 xout", "args mxc_guard net x0 x0 xout", "kernel mxc_guard lists argument x0 twice"),
     "out of order": ("args mxc_call net x0 Ta Pa", "args mxc_call net x0 Pa Ta",

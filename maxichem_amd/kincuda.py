@@ -216,6 +216,11 @@ class CudaSolver(Solver):
         _hip.launch(self._recover, max(1, min(B, self.rec_blocks)), WAVE * _kernels.REC_WAVES, self.lds_rec, stream,
                     rec_args)
 
+    def close(self) -> None:
+        """Unload the kernels and drop the tables and the scratch slots; the counts stay readable."""
+        super().close()
+        self._rscratch = None
+
     def export(self, folder) -> None:
         """Write what a caller without Python needs to run this solver's kernels (maxichem/kincuda.py:363-373): the
         code object, its source, the tables and meta.txt, which mxc_solver_open and maxichem::Solver

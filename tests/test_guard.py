@@ -384,9 +384,10 @@ def test_sum_on_the_tolerance_is_judged_once(ncho):
     assert bool(r.done.all()) and solver.fallback["bad_input"] == 0
 
 
-def test_close_releases_the_gpu_memory(read_network, gpu):
+@pytest.mark.parametrize("variant", [(1, 16, 1, 0), (1, 16, 1, 1)], ids=["rates in LDS", "rates in global scratch"])
+def test_close_releases_the_gpu_memory(read_network, gpu, variant):
     before = torch.cuda.memory_allocated()
-    solver = CudaSolver(read_network("synthetic_small"), "cuda")
+    solver = CudaSolver(read_network("synthetic_small"), "cuda", variant=variant)
     held = torch.cuda.memory_allocated() - before
     solver.close()
     left = torch.cuda.memory_allocated() - before

@@ -352,10 +352,10 @@ def main() -> int:
     forms = ("plain", "mixed", "learned", "learned_mixed") if args.form == "all" else (args.form,)
     if "plain" in forms:
         runs["plain"] = timed(solver, x, T, P, dt, h, J, args.steps, flops_of)
-    slow_steps = min(args.steps, 2)  # the mixed forms take hundreds of times the plain form's tries per call
     if "mixed" in forms:
         mixed = CudaSolver(net, dev, photo=True, precision="mixed")
-        runs["mixed"] = timed(mixed, x, T, P, dt, h, J, slow_steps, flops_of)
+        mixed(x, T, P, dt, h, J=J)  # warm-up (its first call from the same start)
+        runs["mixed"] = timed(mixed, x, T, P, dt, h, J, args.steps, flops_of)
         mixed.close()
     ndata = 0
     if "learned" in forms or "learned_mixed" in forms:
@@ -365,9 +365,8 @@ def main() -> int:
         for name, precision in (("learned", "fp64"), ("learned_mixed", "mixed")):
             if name in forms:
                 ls = LearnedSolver(net, dev, sur, photo=True, precision=precision)
-                if precision == "fp64":
-                    ls(x, T, P, dt, h, J=J)  # warm-up of the learned path (its first call from the same start)
-                runs[name] = timed(ls, x, T, P, dt, h, J, args.steps if precision == "fp64" else slow_steps, flops_of)
+                ls(x, T, P, dt, h, J=J)  # warm-up of the learned path (its first call from the same start)
+                runs[name] = timed(ls, x, T, P, dt, h, J, args.steps, flops_of)
                 ls.close()
     for name, rr in runs.items():
         log(f"{name}: {len(rr['tries'])} timed calls {rr['wall']:.3f} s; call kernel {rr['kernel_ms'].mean():.3f} ms "

@@ -5,6 +5,87 @@
 // K3 = W^-1 (F(N + 2 K1) + (K1 - K2) / h), K4 = W^-1 (F(N + 2 K1 + K3) + (K1 - K2 - 8/3 K3) / h), with
 // W^-1 y = M0^-1 y + c0 (1^T M0^-1 y) M0^-1 w by Sherman-Morrison.
 
+// M0's factorization and its solves, by the trailing block's type: FP64; or, in the mixed form, FP32 with each solve
+// refined in FP64 (false when the refinement did not converge).
+__device__ __forceinline__ void factor_m0(double (&c)[NT], LeadRow& a21, double& dl, double& dtr, double* ws,
+                                          int lane) {
+  factor(c, a21, dl, dtr, ws, lane);
+}
+
+template <int NV>
+__device__ __forceinline__ bool solve_m0(double (&yl)[NV], double (&yt)[NV], const double (&c)[NT],
+                                         const LeadRow& a21, double dl, double dtr, double, double, double,
+                                         const Net*, double* ws, int lane) {
+  solve<NV>(yl, yt, c, a21, dl, dtr, ws, lane);
+  return true;
+}
+
+#if MIXED
+__device__ __forceinline__ void factor_m0(float (&c)[NT], LeadRow& a21, double& dl, float& dtr, double* ws, int lane) {
+  factor_lead(c, a21, dl, ws, lane);
+  factor_trailing32(c, dtr, ws, lane);
+}
+
+template <int NV>
+__device__ __forceinline__ bool solve_m0(double (&yl)[NV], double (&yt)[NV], const float (&c)[NT], const LeadRow& a21,
+                                         double dl, float dtr, double igh, double wl, double wt, const Net* net,
+                                         double* ws, int lane) {
+  return msolve<NV>(yl, yt, c, a21, dl, dtr, igh, wl, wt, net, ws, lane);
+}
+#endif
+
+// One try's stages from the state N at the step ha: F, w and -Jn at N, M0 = I / (gamma ha) - Jn factored, then K1 .. K4
+// (the lead / trailing parts). Returns false when a solve's FP32 refinement did not converge (the mixed form only).
+template <typename T>
+__device__ __forceinline__ bool stages(T (&c)[NT], LeadRow& a21, double Nl, double Nt, double ha, double rtol,
+                                       double atol, const Net* net, double nd, double* ws, const double* rk,
+                                       int lane, double& K1l, double& K1t, double& K3l, double& K3t, double& K4l,
+                                       double& K4t) {
+  const double igh = 1.0 / (GAMMA * ha), ih = 1.0 / ha;
+  double Fl = 0.0, Ft = 0.0, ul = 0.0, ut = 0.0, u0l = 0.0, u0t = 0.0;
+  const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
+  jacobian(c, a21, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
+  add_diagonal(c, igh, ws, lane);
+  double dl;
+  T dtr;
+  factor_m0(c, a21, dl, dtr, ws, lane);
+  const double wl = atol + rtol * fabs(Nl), wt = atol + rtol * fabs(Nt);  // the refinement's error weights
+#define MXC_SOLVE(YL, YT) \
+  if (!solve_m0(YL, YT, c, a21, dl, dtr, igh, wl, wt, net, ws, lane)) return false
+  double y2l[2] = {Fl, ul}, y2t[2] = {Ft, ut};
+  MXC_SOLVE(y2l, y2t);
+  const double Ul = y2l[1], Ut = y2t[1];  // M0^-1 w
+  const double c0 = 1.0 / (1.0 - vsum(Ul, Ut, lane));
+  K1l = y2l[0];
+  K1t = y2t[0];
+  double sk = vsum(K1l, K1t, lane) * c0;
+  K1l += sk * Ul;
+  K1t += sk * Ut;
+  double y1l[1] = {fma(RODAS_C21 * ih, K1l, Fl)}, y1t[1] = {fma(RODAS_C21 * ih, K1t, Ft)};
+  MXC_SOLVE(y1l, y1t);
+  sk = vsum(y1l[0], y1t[0], lane) * c0;
+  const double K2l = y1l[0] + sk * Ul, K2t = y1t[0] + sk * Ut;
+  double Yl = Nl + 2.0 * K1l, Yt = Nt + 2.0 * K1t;
+  rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
+  y1l[0] = fma(ih, K1l - K2l, Fl);
+  y1t[0] = fma(ih, K1t - K2t, Ft);
+  MXC_SOLVE(y1l, y1t);
+  sk = vsum(y1l[0], y1t[0], lane) * c0;
+  K3l = y1l[0] + sk * Ul;
+  K3t = y1t[0] + sk * Ut;
+  Yl = Nl + 2.0 * K1l + K3l;
+  Yt = Nt + 2.0 * K1t + K3t;
+  rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
+  y1l[0] = fma(ih, fma(-RODAS_C43, K3l, K1l - K2l), Fl);
+  y1t[0] = fma(ih, fma(-RODAS_C43, K3t, K1t - K2t), Ft);
+  MXC_SOLVE(y1l, y1t);
+#undef MXC_SOLVE
+  sk = vsum(y1l[0], y1t[0], lane) * c0;
+  K4l = y1l[0] + sk * Ul;
+  K4t = y1t[0] + sk * Ut;
+  return true;
+}
+
 extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_waves_per_eu(CALL_WAVES_PER_SIMD)))
     mxc_call(const Net* __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
              const double* __restrict__ Pa, const double* __restrict__ dta, double* __restrict__ hio,
@@ -22,11 +103,7 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_wa
 #endif
   const int waves = (int)gridDim.x * CPW;
   int q = (int)blockIdx.x * CPW + wv;
-#if MIXED
-  float c[NT];  // the trailing block of M0 and its factors, FP32 in the mixed-precision form
-#else
-  double c[NT];
-#endif
+  double c[NT];  // the trailing block of M0 and its factors (in the mixed form, once a cell's FP32 refinement failed)
   LeadRow a21;  // the trailing row's lead columns of M0, then of L
   while (q < B) {
     const int cell = order ? order[q] : q;
@@ -61,61 +138,27 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_wa
         Nt = lane < NT ? xq[perm[K0 + lane]] : 0.0;
       }
     }
+    // the mixed form starts each cell's call in FP32; a refinement that does not converge puts the cell on FP64
+    [[maybe_unused]] bool fp32 = true;
     while (it < max_steps && s.t < dt * CALL_END) {
-      const double ha = verifying ? dt : fmin(s.h, dt - s.t), igh = 1.0 / (GAMMA * ha), ih = 1.0 / ha;
-      // F, w and -Jn at N; a try after a rejection is at the same N and computes the same values again
-      double Fl = 0.0, Ft = 0.0, ul = 0.0, ut = 0.0, u0l = 0.0, u0t = 0.0;
-      const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
-      jacobian(c, a21, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
-      add_diagonal(c, igh, ws, lane);
-      double dl;
+      const double ha = verifying ? dt : fmin(s.h, dt - s.t);
+      double K1l, K1t, K3l, K3t, K4l, K4t;
 #if MIXED
-      float dtr;
-      factor_lead(c, a21, dl, ws, lane);
-      factor_trailing32(c, dtr, ws, lane);
-      const double wl = atol + rtol * fabs(Nl), wt = atol + rtol * fabs(Nt);  // the refinement's error weights
-      bool lin_ok = true;  // every solve's refinement converged; else the try is rejected as a non-finite one
-#define MXC_SOLVE(NV, YL, YT) lin_ok = msolve<NV>(YL, YT, c, a21, dl, dtr, igh, wl, wt, net, ws, lane) && lin_ok
-#else
-      double dtr;
-      factor(c, a21, dl, dtr, ws, lane);
-      constexpr bool lin_ok = true;
-#define MXC_SOLVE(NV, YL, YT) solve<NV>(YL, YT, c, a21, dl, dtr, ws, lane)
-#endif
-      double y2l[2] = {Fl, ul}, y2t[2] = {Ft, ut};
-      MXC_SOLVE(2, y2l, y2t);
-      const double Ul = y2l[1], Ut = y2t[1];  // M0^-1 w
-      const double c0 = 1.0 / (1.0 - vsum(Ul, Ut, lane));
-      double K1l = y2l[0], K1t = y2t[0];
-      double sk = vsum(K1l, K1t, lane) * c0;
-      K1l += sk * Ul;
-      K1t += sk * Ut;
-      double y1l[1] = {fma(RODAS_C21 * ih, K1l, Fl)}, y1t[1] = {fma(RODAS_C21 * ih, K1t, Ft)};
-      MXC_SOLVE(1, y1l, y1t);
-      sk = vsum(y1l[0], y1t[0], lane) * c0;
-      const double K2l = y1l[0] + sk * Ul, K2t = y1t[0] + sk * Ut;
-      double Yl = Nl + 2.0 * K1l, Yt = Nt + 2.0 * K1t;
-      rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
-      y1l[0] = fma(ih, K1l - K2l, Fl);
-      y1t[0] = fma(ih, K1t - K2t, Ft);
-      MXC_SOLVE(1, y1l, y1t);
-      sk = vsum(y1l[0], y1t[0], lane) * c0;
-      const double K3l = y1l[0] + sk * Ul, K3t = y1t[0] + sk * Ut;
-      Yl = Nl + 2.0 * K1l + K3l;
-      Yt = Nt + 2.0 * K1t + K3t;
-      rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
-      y1l[0] = fma(ih, fma(-RODAS_C43, K3l, K1l - K2l), Fl);
-      y1t[0] = fma(ih, fma(-RODAS_C43, K3t, K1t - K2t), Ft);
-      MXC_SOLVE(1, y1l, y1t);
-      sk = vsum(y1l[0], y1t[0], lane) * c0;
-      const double K4l = y1l[0] + sk * Ul, K4t = y1t[0] + sk * Ut;
-#undef MXC_SOLVE
-      bool ok = false;
-      double en = try_error(Nl, Nt, K1l, K1t, K3l, K3t, K4l, K4t, Yl, Yt, s.t == 0.0, rtol, atol, lane, ok);
-      if (!lin_ok) {
-        ok = false;
-        en = DNAN;
+      if (fp32) {
+        float cf[NT];  // the trailing block in FP32
+        if (!stages(cf, a21, Nl, Nt, ha, rtol, atol, net, nd, ws, rk, lane, K1l, K1t, K3l, K3t, K4l, K4t)) {
+          fp32 = false;  // this try again in FP64, and the cell's later tries of the call
+          continue;
+        }
+      } else {
+        stages(c, a21, Nl, Nt, ha, rtol, atol, net, nd, ws, rk, lane, K1l, K1t, K3l, K3t, K4l, K4t);
       }
+#else
+      stages(c, a21, Nl, Nt, ha, rtol, atol, net, nd, ws, rk, lane, K1l, K1t, K3l, K3t, K4l, K4t);
+#endif
+      bool ok = false;
+      double Yl, Yt;
+      const double en = try_error(Nl, Nt, K1l, K1t, K3l, K3t, K4l, K4t, Yl, Yt, s.t == 0.0, rtol, atol, lane, ok);
       if (verifying) {
         // the proposal is kept when the try passes and moves it by no more than the tolerance: the end state is
         // then stationary over dt; otherwise the cell starts again from its input

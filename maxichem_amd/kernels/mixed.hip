@@ -1,7 +1,9 @@
 // The mixed-precision form: the trailing block of M0 = I / (gamma h) - Jn is factored and its triangular solves run in
 // FP32; each solve of M0 y = b is then refined with residuals r = b - M0 y taken in FP64 without the matrix, from the
-// dv terms of the try (M0 y = y / (gamma h) - S (dv/dn y)), at most REFINE_SWEEPS times. The rates, the right-hand
-// sides, the lead rows of the factorization (LDS), the error estimate and the guard stay FP64.
+// dv terms of the try (M0 y = y / (gamma h) - S (dv/dn y)), at most REFINE_SWEEPS times. A refinement that does not
+// converge sends the try, and the cell's later tries of the call, to the FP64 factorization (the call kernel's
+// fallback, as LAPACK's dsgesv). The rates, the right-hand sides, the lead rows of the factorization (LDS), the error
+// estimate and the guard stay FP64.
 
 constexpr int REFINE_SWEEPS = 3;
 constexpr double REFINE_TOL = 1e-4;  // a sweep's correction below this many error weights ends the refinement
@@ -147,7 +149,7 @@ __device__ __forceinline__ void m0_times(const double (&xl)[NV], const double (&
 // M0^-1 b for NV vectors with the FP32 trailing factors and FP64 refinement; wl, wt: the error weights
 // atol + rtol |N| of the try's state. b comes in yl, yt and the solution goes out there. Returns whether the last
 // sweep's correction fell below REFINE_TOL weights: where M0 is too ill-conditioned for FP32 (large steps of stiff
-// chemistry) the refinement stalls or diverges, and the try is then not to be trusted.
+// chemistry) the refinement stalls or diverges, and the try then goes to FP64.
 template <int NV>
 __device__ __forceinline__ bool msolve(double (&yl)[NV], double (&yt)[NV], const float (&c)[NT], const LeadRow& a21,
                                        double dl, float dt, double igh, double wl, double wt, const Net* net,

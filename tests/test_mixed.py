@@ -2,9 +2,10 @@
 the FP64 oracle to the parity bar (species above 1e-20 within relative 1e-2) with the FP64 form's done flags; each
 form's error against the oracle and against each other, and the tries each takes, are printed.
 
-A try whose refinement does not converge is rejected (the step shrinks), so the mixed form keeps the method's
-accuracy at the price of tries: on stiff chemistry at steps of 1e2 s and more, M0's condition number (1e9 to 1e17)
-is beyond what FP32 factors can refine."""
+A try whose FP32 refinement does not converge is done again in FP64, and so are the cell's later tries of the call
+(a mixed-precision solver's fallback, as LAPACK's dsgesv): on stiff chemistry at steps of 1e2 s and more, M0's
+condition number (1e9 to 1e17) is beyond what FP32 factors can refine, and such cells end in FP64. The tries are the
+FP64 form's, give or take the rounding of the tries that FP32 kept."""
 
 import numpy as np
 import oracle

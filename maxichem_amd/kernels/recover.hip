@@ -5,11 +5,13 @@
 // Dense LU of W (NS x NS, row-major, LDS) in place, shared by the workgroup's REC_WAVES wavefronts (wv: this one's
 // index): row k is exchanged with the row of the largest |entry| of column k (ties to the lower row; a NaN is never
 // chosen) only when |W_kk| < PIVTAU times it or is not finite. pr[i]: the original row now in row i; dinv[i] =
-// 1 / U_ii. Every wavefront finds the pivot itself from the same column, so all take the same branches and meet at
-// the same barriers: the largest entry from a wave reduction of the lanes' own largest, then its lowest row from a
-// second one. Lane l updates rows k + 1 + l (+ WAVE); wavefront wv the blocks of REC_BLOCK columns b = wv, wv +
+// 1 / U_ii. Every wavefront reads column k once (rows k + 1 + l (+ WAVE) in lane l's registers, W_kk in all lanes)
+// and finds the pivot itself from it, so all take the same branches and meet at the same barriers: the largest
+// entry from a wave reduction of the lanes' own largest, then its lowest row from a second one. The pivot and the
+// multipliers come from those registers. Wavefront wv updates the blocks of REC_BLOCK columns b = wv, wv +
 // REC_WAVES, ... from column k + 1, so every entry takes the same operations in the same order as with one
-// wavefront.
+// wavefront. Wavefront 0 writes column k's multipliers at the start of column k + 1: no other wavefront reads
+// column k again, and the next exchange, which moves them, comes after a barrier.
 constexpr int REC_BLOCK = 8;
 constexpr int REC_ROWS = (NS + WAVE - 2) / WAVE;  // rows below a pivot per lane
 
@@ -17,19 +19,40 @@ __device__ void dense_lu(double* W, int* pr, double* dinv, int lane, int wv) {
   if (wv == 0)
     for (int i = lane; i < NS; i += WAVE) pr[i] = i;
   __syncthreads();
+  double l[REC_ROWS];  // column k's multipliers, rows k + 1 + lane + WAVE p
   for (int k = 0; k < NS; k++) {
+    if (wv == 0 && k > 0) {
+#pragma unroll
+      for (int p = 0; p < REC_ROWS; p++) {
+        const int i = k + lane + WAVE * p;  // rows below pivot k - 1
+        if (i < NS) W[i * NS + k - 1] = l[p];
+      }
+    }
+    double v[REC_ROWS];
+#pragma unroll
+    for (int p = 0; p < REC_ROWS; p++) {
+      const int i = k + 1 + lane + WAVE * p;
+      v[p] = i < NS ? W[i * NS + k] : 0.0;
+    }
+    const double akk = W[k * NS + k];
     double best = -1.0;
     int bi = NS;
-    for (int i = k + lane; i < NS; i += WAVE) {
-      const double a = fabs(W[i * NS + k]);
-      if (a > best) {
-        best = a;
+    if (lane == 0 && fabs(akk) > best) {  // row k first: each lane sees its rows in increasing order
+      best = fabs(akk);
+      bi = k;
+    }
+#pragma unroll
+    for (int p = 0; p < REC_ROWS; p++) {
+      const int i = k + 1 + lane + WAVE * p;
+      if (i < NS && fabs(v[p]) > best) {
+        best = fabs(v[p]);
         bi = i;
       }
     }
     const double top = wave_max(best);
     bi = wave_min(best == top ? bi : NS);
-    const bool exchange = bi != NS && bi != k && !(fabs(W[k * NS + k]) >= PIVTAU * top);
+    const bool exchange = bi != NS && bi != k && !(fabs(akk) >= PIVTAU * top);
+    double piv = akk;
     if (exchange) {
       __syncthreads();  // every wavefront has read column k
       for (int j = lane + WAVE * wv; j < NS; j += WAVE * REC_WAVES) {
@@ -43,21 +66,22 @@ __device__ void dense_lu(double* W, int* pr, double* dinv, int lane, int wv) {
         pr[bi] = t;
       }
       __syncthreads();
+      // row bi's entry is the pivot now, and row bi holds row k's
+      const int ob = bi - k - 1, lb = ob % WAVE, pb = ob / WAVE;
+      double vb = v[0];
+#pragma unroll
+      for (int p = 1; p < REC_ROWS; p++) vb = pb == p ? v[p] : vb;
+      piv = readlane(vb, lb);
+#pragma unroll
+      for (int p = 0; p < REC_ROWS; p++) v[p] = (p == pb && lane == lb) ? akk : v[p];
     }
-    const double dk = 1.0 / W[k * NS + k];
+    const double dk = 1.0 / piv;
     if (wv == 0 && lane == 0) dinv[k] = dk;
-    double l[REC_ROWS];
 #pragma unroll
     for (int p = 0; p < REC_ROWS; p++) {
       const int i = k + 1 + lane + WAVE * p;
-      l[p] = i < NS ? W[i * NS + k] * dk : 0.0;
-    }
-    __syncthreads();  // every wavefront has read column k before the first writes its multipliers
-#pragma unroll
-    for (int p = 0; p < REC_ROWS; p++) {
-      const int i = k + 1 + lane + WAVE * p;
+      l[p] = i < NS ? v[p] * dk : 0.0;
       if (i < NS) {
-        if (wv == 0) W[i * NS + k] = l[p];
         for (int j0 = k + 1 + REC_BLOCK * wv; j0 < NS; j0 += REC_BLOCK * REC_WAVES) {
           if (j0 + REC_BLOCK <= NS) {  // a whole block: its loads go out together
             double u[REC_BLOCK], w[REC_BLOCK];
@@ -74,7 +98,7 @@ __device__ void dense_lu(double* W, int* pr, double* dinv, int lane, int wv) {
         }
       }
     }
-    __syncthreads();  // the updates are in before the next column is searched
+    __syncthreads();  // the updates are in before the next column is read
   }
 }
 

@@ -64,6 +64,22 @@ __device__ __forceinline__ double wave_sum(double v) {
   return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
 }
 
+// N sums at once, each as wave_sum (the same bits): every DPP step is taken for all N values before the next, so
+// the N reductions overlap instead of waiting out each other's latency.
+template <int N>
+__device__ __forceinline__ void wave_sums(double (&v)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; i++) v[i] += dpp<DPP_XOR1>(v[i]);
+#pragma unroll
+  for (int i = 0; i < N; i++) v[i] += dpp<DPP_XOR2>(v[i]);
+#pragma unroll
+  for (int i = 0; i < N; i++) v[i] += dpp<DPP_HALF_MIRROR>(v[i]);
+#pragma unroll
+  for (int i = 0; i < N; i++) v[i] += dpp<DPP_MIRROR>(v[i]);
+#pragma unroll
+  for (int i = 0; i < N; i++) v[i] = (readlane(v[i], 0) + readlane(v[i], 16)) + (readlane(v[i], 32) + readlane(v[i], 48));
+}
+
 __device__ __forceinline__ double wave_max(double v) {
   v = fmax(v, dpp<DPP_XOR1>(v));
   v = fmax(v, dpp<DPP_XOR2>(v));
@@ -362,12 +378,13 @@ __device__ __forceinline__ void jacobian(T (&c)[NT], LeadRow& a21, double Nl, do
   if constexpr (K0 > 0 && K0 <= SMALL_LEAD) {
     // few lead rows: each row times N as this lane's share (its own N entries), summed over the lanes; the trailing
     // rows' lead columns times N with N_k by v_readlane
+    double sum[K0 > 0 ? K0 : 1];
 #pragma unroll
-    for (int i = 0; i < K0; i++) {
-      const double part = (lane < K0 ? ws[i * NS + lane] * Nl : 0.0) + (lane < NT ? ws[i * NS + K0 + lane] * Nt : 0.0);
-      const double sum = wave_sum(part);
-      if (lane == i) ml = sum;
-    }
+    for (int i = 0; i < K0; i++)
+      sum[i] = (lane < K0 ? ws[i * NS + lane] * Nl : 0.0) + (lane < NT ? ws[i * NS + K0 + lane] * Nt : 0.0);
+    wave_sums(sum);
+#pragma unroll
+    for (int i = 0; i < K0; i++) ml = lane == i ? sum[i] : ml;
   } else if (K0 > 0) {
 #pragma unroll 1
     for (int j0 = 0; j0 < NS; j0 += STREAM_BATCH) {  // the lead rows, a batch of columns at a time
@@ -602,14 +619,18 @@ __device__ __forceinline__ void solve_lead_backward(double (&yl)[NV], const doub
                                                     int lane) {
   if constexpr (K0 > 0 && K0 <= SMALL_LEAD) {
     // U of the lead rows, trailing columns: lane t's share of each lead row, summed over the lanes
+    double sum[K0 > 0 ? K0 * NV : 1];
 #pragma unroll
     for (int i = 0; i < K0; i++) {
       const double a = lane < NT ? ws[i * NS + K0 + lane] : 0.0;
 #pragma unroll
-      for (int v = 0; v < NV; v++) {
-        const double sum = wave_sum(a * yt[v]);
-        if (lane == i) yl[v] -= sum;
-      }
+      for (int v = 0; v < NV; v++) sum[i * NV + v] = a * yt[v];
+    }
+    wave_sums(sum);
+#pragma unroll
+    for (int i = 0; i < K0; i++) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) yl[v] = lane == i ? yl[v] - sum[i * NV + v] : yl[v];
     }
 #pragma unroll
     for (int k = K0 - 1; k >= 0; k--) {  // U, lead columns

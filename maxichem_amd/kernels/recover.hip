@@ -102,31 +102,40 @@ __device__ void dense_lu(double* W, int* pr, double* dinv, int lane, int wv) {
   }
 }
 
-// y <- W^-1 y with dense_lu's factors; y is the vector in LDS (elimination order). Unrolled with every lane's factor
-// entry loaded in bounds and its update kept by a select, so the chain from step to step is a readlane and an fma.
+// y <- W^-1 y with dense_lu's factors; y is the vector in LDS (elimination order). Up to ROW_REGISTERS species, this
+// lane's row of the factors (row lane, clamped in bounds) is loaded into registers before the substitutions, so the
+// chain from step to step is a readlane and an fma with no LDS wait (beyond that the row would not fit the
+// registers, and its entries are read in the steps). Rows WAVE and beyond (a second row per lane) read their entries
+// in the steps. Every update is kept by a select, as the substitutions' order fixes the bits.
+constexpr int ROW_REGISTERS = 80;
+constexpr bool ROW_IN_REGISTERS = NS <= ROW_REGISTERS;
+
 __device__ void dense_solve(const double* W, const int* pr, const double* dinv, double* y, int lane) {
   double a[2];
 #pragma unroll
   for (int p = 0; p < 2; p++) a[p] = lane + WAVE * p < NS ? y[pr[lane + WAVE * p]] : 0.0;
-  const int r0 = lane, r1 = lane + WAVE < NS ? lane + WAVE : NS - 1;  // rows in bounds (the second masked off)
+  const int r0 = lane < NS ? lane : NS - 1, r1 = lane + WAVE < NS ? lane + WAVE : NS - 1;  // rows in bounds
+  double w0[NS];
+  if constexpr (ROW_IN_REGISTERS) {
+#pragma unroll
+    for (int k = 0; k < NS; k++) w0[k] = W[r0 * NS + k];
+  }
+  const double d0 = dinv[r0], d1 = dinv[r1];
 #pragma unroll
   for (int k = 0; k < NS - 1; k++) {
     const double yk = readlane(k < WAVE ? a[0] : a[1], k & (WAVE - 1));
-    const double w0 = W[r0 * NS + k], w1 = W[r1 * NS + k];
-    a[0] = lane > k ? fma(-w0, yk, a[0]) : a[0];
-    if (NS > WAVE) a[1] = lane + WAVE > k && lane + WAVE < NS ? fma(-w1, yk, a[1]) : a[1];
+    a[0] = lane > k ? fma(-(ROW_IN_REGISTERS ? w0[k] : W[r0 * NS + k]), yk, a[0]) : a[0];
+    if (NS > WAVE) a[1] = lane + WAVE > k && lane + WAVE < NS ? fma(-W[r1 * NS + k], yk, a[1]) : a[1];
   }
 #pragma unroll
   for (int k = NS - 1; k > 0; k--) {
-    const double zk = readlane(k < WAVE ? a[0] : a[1], k & (WAVE - 1)) * dinv[k];
-    const double w0 = W[r0 * NS + k], w1 = W[r1 * NS + k];
-    a[0] = lane < k ? fma(-w0, zk, a[0]) : a[0];
-    if (NS > WAVE) a[1] = lane + WAVE < k ? fma(-w1, zk, a[1]) : a[1];
+    const double zk = readlane(k < WAVE ? a[0] : a[1], k & (WAVE - 1)) * readlane(k < WAVE ? d0 : d1, k & (WAVE - 1));
+    a[0] = lane < k ? fma(-(ROW_IN_REGISTERS ? w0[k] : W[r0 * NS + k]), zk, a[0]) : a[0];
+    if (NS > WAVE) a[1] = lane + WAVE < k ? fma(-W[r1 * NS + k], zk, a[1]) : a[1];
   }
   wave_sync();
-#pragma unroll
-  for (int p = 0; p < 2; p++)
-    if (lane + WAVE * p < NS) y[lane + WAVE * p] = a[p] * dinv[lane + WAVE * p];
+  if (lane < NS) y[lane] = a[0] * d0;
+  if (lane + WAVE < NS) y[lane + WAVE] = a[1] * d1;
   wave_sync();
 }
 

@@ -400,6 +400,9 @@ class LearnedSolver(CudaSolver):
     Proposals are off by default: a try from a state tests that the state is stationary over dt, not that it is the
     cell's state at the end of dt, and where chemistry is slow a wrong proposal passes (measured on the NCHO
     workload: species off by several times).
+
+    A call with warm="cold", no proposals and the previous call's tries to order the queue runs the inference on its
+    cold cells' rows only (none if no cell is cold); predicted_cost is then None.
     """
 
     COLD = 1e-6
@@ -413,6 +416,19 @@ class LearnedSolver(CudaSolver):
         self.predicted_cost: torch.Tensor | None = None
 
     def _plan(self, x, T, P, dt, h_start, jx, B):
+        prev = self.tries
+        measured = prev is not None and prev.shape[0] == B
+        if self.warm == "cold" and not self.propose and measured:
+            # only the cold cells' first steps come from the prediction (the queue follows the measured tries):
+            # inference on their rows alone, which give the same values as in a whole batch (one host sync)
+            cold = ~torch.isfinite(h_start) | (h_start < self.COLD * dt)
+            idx = torch.nonzero(cold).squeeze(1)
+            if idx.numel():
+                h0, _, _ = self.surrogate.predict(x[idx], T[idx], P[idx], dt[idx], None if jx is None else jx[idx],
+                                                  H_MIN, proposals=False)  # fmt: skip
+                h_start[idx] = h0
+            self.predicted_cost = None
+            return torch.argsort(prev, descending=True).to(torch.int32), None
         h0, cost, xp = self.surrogate.predict(x, T, P, dt, jx, H_MIN, proposals=self.propose)
         if self.warm == "all":
             h_start.copy_(h0)

@@ -48,7 +48,7 @@ KERNEL_NAMES = ("mxc_call", "mxc_guard", "mxc_recover")
 # the reference's KERNEL_ARGUMENTS (maxichem/_constants.py:148-164) does for its kernels.
 KERNEL_ARGUMENTS = {
     "mxc_call": ("net", "x0", "Ta", "Pa", "dta", "hio", "xout", "doneo", "noko", "status", "count", "tries", "order",
-                 "Jin", "B", "rtol", "atol", "max_steps", "xprop", "rscratch"),
+                 "Jin", "B", "rtol", "atol", "max_steps", "xprop", "rscratch", "fp64"),
     "mxc_guard": ("net", "x0", "xout", "doneo", "noko", "status", "redo", "list", "count", "B", "atol"),
     "mxc_recover": ("net", "x0", "Ta", "Pa", "dta", "hin", "xout", "hio", "doneo", "noko", "redo", "list", "count",
                     "tries", "Jin", "rtol", "atol", "max_steps", "rscratch"),

@@ -38,7 +38,7 @@ struct KernelArguments {
 };
 const KernelArguments kKernels[] = {
     {"mxc_call", {"net", "x0", "Ta", "Pa", "dta", "hio", "xout", "doneo", "noko", "status", "count", "tries", "order",
-                  "Jin", "B", "rtol", "atol", "max_steps", "xprop", "rscratch"}},
+                  "Jin", "B", "rtol", "atol", "max_steps", "xprop", "rscratch", "fp64"}},
     {"mxc_guard", {"net", "x0", "xout", "doneo", "noko", "status", "redo", "list", "count", "B", "atol"}},
     {"mxc_recover", {"net", "x0", "Ta", "Pa", "dta", "hin", "xout", "hio", "doneo", "noko", "redo", "list", "count",
                      "tries", "Jin", "rtol", "atol", "max_steps", "rscratch"}},
@@ -484,6 +484,7 @@ int mxc_solver_call(mxc_solver* s, void* stream_, int cells, const double* x, co
     return 1;
   const int* order = nullptr;  // cells in index order (results do not depend on the order)
   const double* xprop = nullptr;
+  int* fp64 = nullptr;  // the mixed form's cells that fell back to FP64 in an earlier call: none kept here
   int n = cells, rec_steps = s->rec_steps * max_steps;
   // argument values by name, in each kernel's order
   std::map<std::string, void*> v{
@@ -491,7 +492,7 @@ int mxc_solver_call(mxc_solver* s, void* stream_, int cells, const double* x, co
       {"hin", &h},        {"hio", &h_out},  {"xout", &x_out},  {"doneo", &done},    {"noko", &steps},
       {"status", &s->status}, {"redo", &redo}, {"list", &s->list}, {"count", &s->count}, {"tries", &s->tries},
       {"order", &order},  {"Jin", &J},      {"B", &n},         {"rtol", &rtol},     {"atol", &atol},
-      {"max_steps", &max_steps}, {"xprop", &xprop}, {"rscratch", &s->rscratch}};
+      {"max_steps", &max_steps}, {"xprop", &xprop}, {"rscratch", &s->rscratch}, {"fp64", &fp64}};
   auto launch = [&](int k, hipFunction_t f, unsigned grid, unsigned block, unsigned lds) {
     std::vector<void*> a;
     for (const auto& name : kKernels[k].names) a.push_back(v.at(name));

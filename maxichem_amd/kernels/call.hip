@@ -92,7 +92,8 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_wa
              double* __restrict__ xout, int* __restrict__ doneo, double* __restrict__ noko,
              int* __restrict__ status, int* __restrict__ count, double* __restrict__ tries,
              const int* __restrict__ order, const double* __restrict__ Jin, int B, double rtol, double atol,
-             int max_steps, const double* __restrict__ xprop, double* __restrict__ rscratch) {
+             int max_steps, const double* __restrict__ xprop, double* __restrict__ rscratch,
+             int* __restrict__ fp64) {
   extern __shared__ double lds[];
   const int lane = (int)(threadIdx.x & (WAVE - 1)), wv = (int)(threadIdx.x / WAVE);
   double* ws = lds + wv * L_SIZE;
@@ -138,8 +139,9 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_wa
         Nt = lane < NT ? xq[perm[K0 + lane]] : 0.0;
       }
     }
-    // the mixed form starts each cell's call in FP32; a refinement that does not converge puts the cell on FP64
-    [[maybe_unused]] bool fp32 = true;
+    // the mixed form starts a cell's call in FP32, unless fp64 (per cell; may be null) says that an earlier call fell
+    // back for it; a refinement that does not converge puts the cell on FP64, and marks it there
+    [[maybe_unused]] bool fp32 = !(fp64 != nullptr && fp64[cell] != 0);
     while (it < max_steps && s.t < dt * CALL_END) {
       const double ha = verifying ? dt : fmin(s.h, dt - s.t);
       double K1l, K1t, K3l, K3t, K4l, K4t;
@@ -148,6 +150,7 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_wa
         float cf[NT];  // the trailing block in FP32
         if (!stages(cf, a21, Nl, Nt, ha, rtol, atol, net, nd, ws, rk, lane, K1l, K1t, K3l, K3t, K4l, K4t)) {
           fp32 = false;  // this try again in FP64, and the cell's later tries of the call
+          if (lane == 0 && fp64 != nullptr) fp64[cell] = 1;
           continue;
         }
       } else {

@@ -131,6 +131,7 @@ class CudaSolver(Solver):
         self._net = torch.tensor([self._tables[k].data_ptr() for k in _kernels.NET_FIELDS], dtype=torch.int64,
                                  device=self.dev)  # fmt: skip
         self.tries: torch.Tensor | None = None
+        self._fp64: torch.Tensor | None = None  # the mixed form's cells that fell back to FP64 (_fp64_cells)
         # a list to collect (start, end) events of each call kernel launch on the call's stream; None: not recorded
         self.timing: list | None = None
         self.info = {
@@ -197,7 +198,7 @@ class CudaSolver(Solver):
         net = p(self._net)
         call_args = [net, p(x), p(T), p(P), p(dt), p(ho), p(xo), p(done), p(nok), p(status), p(self.count),
                      p(self.tries), p(order), p(jx), ctypes.c_int(B), ctypes.c_double(rtol), ctypes.c_double(atol),
-                     ctypes.c_int(max_steps), p(xprop), p(self._rscratch)]  # fmt: skip
+                     ctypes.c_int(max_steps), p(xprop), p(self._rscratch), p(self._fp64_cells(B))]  # fmt: skip
         grid = max(1, min(-(-B // self.cpw), self.max_blocks))
         if self.timing is not None:
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -216,10 +217,21 @@ class CudaSolver(Solver):
         _hip.launch(self._recover, max(1, min(B, self.rec_blocks)), WAVE * _kernels.REC_WAVES, self.lds_rec, stream,
                     rec_args)
 
+    def _fp64_cells(self, B: int) -> torch.Tensor | None:
+        """The mixed form's per-cell flags [B] (int32): the cells whose FP32 refinement failed in an earlier call with
+        as many cells, which then start in FP64 (a call usually repeats the previous call's cells, as for the queue's
+        order); None in the FP64 form."""
+        if self.precision != "mixed":
+            return None
+        if self._fp64 is None or self._fp64.shape[0] != B:
+            self._fp64 = torch.zeros(B, dtype=torch.int32, device=self.dev)
+        return self._fp64
+
     def close(self) -> None:
         """Unload the kernels and drop the tables and the scratch slots; the counts stay readable."""
         super().close()
         self._rscratch = None
+        self._fp64 = None
 
     def export(self, folder) -> None:
         """Write what a caller without Python needs to run this solver's kernels (maxichem/kincuda.py:363-373): the

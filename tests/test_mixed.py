@@ -65,3 +65,7 @@ def test_mixed_matches_fp64_and_the_oracle(shape):
                 assert float(mix.tries.sum()) <= 1.02 * float(f64.tries.sum())
             x, h = xa, a.h.cpu().numpy()  # both continue from the FP64 state
         print(f"{shape}: tries over the calls FP64 {tf:.0f}, mixed {tm:.0f}")
+        # the cells whose FP32 refinement failed are kept, per cell, for the next call over as many cells
+        marked = mix._fp64_cells(1024)
+        print(f"{shape}: {int(marked.sum())} of 1024 cells start in FP64")
+        assert marked.dtype == torch.int32 and 0 < int(marked.sum()) <= 1024 and f64._fp64_cells(1024) is None

@@ -124,3 +124,24 @@ def test_chunks_stage_the_trailing_rows_by_column(read_network):
                     if i >= tab.k0 and i != j:
                         want.add((i, j))
         assert got == want
+
+
+def test_tables_refuse_networks_beyond_the_kernels_limits(read_network):
+    # one wavefront per cell holds at most 128 species (the lead part one register per lane), a reaction at most
+    # three species a side, and a stoichiometric coefficient at most 2: each refused by name before any device work
+    import copy
+    import types
+
+    net = read_network("synthetic_small")
+    fields = {k: getattr(net, k) for k in ("reac", "photo", "elements", "atoms", "lo", "hi", "species", "S", "Sp")}
+    big = types.SimpleNamespace(**dict(fields, S=np.zeros((_tables.MAX_SPECIES + 1, len(net.reac)))))
+    with pytest.raises(ValueError, match=f"at most {_tables.MAX_SPECIES} species, one wavefront per cell"):
+        _tables.build(big)
+    reac = copy.deepcopy(list(net.reac))
+    reac[0]["re"] = list(reac[0]["re"]) + [0] * (4 - len(reac[0]["re"]))
+    with pytest.raises(ValueError, match="at most 3 species per side"):
+        _tables.build(types.SimpleNamespace(**dict(fields, reac=reac)))
+    S = np.array(net.S, dtype=np.float64)
+    S[0, 0] = 3.0
+    with pytest.raises(ValueError, match=r"\|S\| <= 2 per species and reaction"):
+        _tables.build(types.SimpleNamespace(**dict(fields, S=S)))

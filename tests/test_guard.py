@@ -402,3 +402,18 @@ def test_an_empty_batch_is_a_call(ncho):
     r = solver.launch(torch.empty(0, ns, dtype=torch.float64, device="cuda"), e, e, e, e)
     assert r.x.shape == (0, ns) and r.h.shape == (0,) and r.done.shape == (0,) and r.flag.shape == (0,)
     assert int(solver.count[1:-1].abs().sum()) == 0
+
+
+def test_an_empty_j_is_none_without_photolysis(read_network, gpu):
+    # without photolysis, a J of no columns is the same call as no J; one with columns is refused by name
+    net = read_network("synthetic_small")
+    cells = 4
+    x = torch.zeros(cells, len(net.species), dtype=torch.float64, device=gpu)
+    for name, value in (("H", 0.1), ("H2", 0.6), ("OH", 0.05), ("H2O", 0.1), ("O", 0.05), ("He", 0.1)):
+        x[:, net.species.index(name)] = value
+    T, P, dt, h = (torch.full((cells,), q, dtype=torch.float64, device=gpu) for q in (1500.0, 1e6, 1e-3, 1e-3))
+    empty = torch.zeros(cells, 0, dtype=torch.float64, device=gpu)
+    with CudaSolver(net, gpu) as solver:
+        assert all(map(torch.equal, solver(x, T, P, dt, h), solver(x, T, P, dt, h, J=empty)))
+        with pytest.raises(ValueError, match=r"^J: none or an empty \[4, 0\] one expected"):
+            solver(x, T, P, dt, h, J=torch.zeros(cells, 1, dtype=torch.float64, device=gpu))

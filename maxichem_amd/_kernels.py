@@ -54,7 +54,7 @@ KERNEL_ARGUMENTS = {
                     "tries", "Jin", "rtol", "atol", "max_steps", "rscratch"),
 }  # fmt: skip
 # The kernels' Net struct (kernels/common.hip): one device pointer per table, in this order.
-NET_FIELDS = ("perm", "rc", "rf", "nasa", "rxe", "svw", "rowp", "diagw", "leadw", "chw", "atoms")
+NET_FIELDS = ("perm", "rc", "rf", "nasa", "rxe", "svw", "rowp", "diagw", "leadw", "chw", "atoms", "rord")
 
 # The method's settings the kernels read as constexprs of the same names.
 _SETTINGS = (
@@ -138,6 +138,7 @@ def host_tables(tab: Tables) -> dict[str, np.ndarray]:
         "leadw": tab.lead_words.ravel().view(np.int32),
         "chw": np.concatenate([w.ravel() for w in tab.chunk_words]).view(np.int32),
         "atoms": tab.atoms.astype(np.float64),
+        "rord": tab.rord.astype(np.int32),
     }
     return {k: np.ascontiguousarray(v) for k, v in t.items()}
 

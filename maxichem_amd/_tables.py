@@ -147,6 +147,7 @@ class Tables:
     has_eff: bool
     # kernel streams
     rxe: np.ndarray = field(default=None)  # uint64 [nr]: 6 elimination-order species (8 bits, pad ns), dv offset
+    rord: np.ndarray = field(default=None)  # int32 [nr]: the rate coefficients' order, reactions grouped by law
     ndv: int = 0
     diag_words: np.ndarray = field(default=None)  # uint32 [rounds, 64]
     lead_words: np.ndarray = field(default=None)  # uint32 [rounds, 64]
@@ -272,6 +273,11 @@ def build(net: VulcanNetwork, photo: bool = False) -> Tables:
         w |= int(rdv[r]) << 48
         rxe[r] = w
     tab.rxe = rxe
+    # the rate coefficients are computed in this order, 64 reactions a round: grouped by law (Troe falloff, falloff,
+    # a second low-pressure term, plain Arrhenius, photolysis), a round takes the code of fewer laws
+    law = np.where(rf & FL_PHOTO, 4, np.where(rf & FL_TROE, 0, np.where(rf & FL_FALLOFF, 1,
+                                                                          np.where(rf & FL_SECOND_K0, 2, 3))))
+    tab.rord = np.argsort(law, kind="stable").astype(np.int32)
     lay = tab.layout()
     reach = max(lay["STAGE"] + nt * CHUNK, lay["DIAG"] + nt)  # the furthest LDS offset a stream word addresses
     if reach >= (1 << T_DEST_BITS):

@@ -29,8 +29,16 @@ namespace {
 constexpr int kExportFormat = 2;  // 2: the recovery workgroup's size (rec_threads)
 constexpr int kWave = 64;  // lanes of a wavefront: one wavefront steps one cell
 // The kernels' Net struct, one device pointer per table in this order (kernels/common.hip).
-const char* const kNetFields[] = {"perm", "rc", "rf", "nasa", "rxe", "svw", "rowp", "diagw", "leadw", "chw", "atoms"};
+const char* const kNetFields[] = {"perm",  "rc",   "rf",  "nasa",  "rxe",   "svw",
+                                  "rowp",  "diagw", "leadw", "chw", "atoms", "rord"};
 constexpr int kNetCount = sizeof(kNetFields) / sizeof(kNetFields[0]);
+
+// The position of a Net field by name (the guard's atoms table is read on the host as well).
+int net_field(const char* name) {
+  for (int i = 0; i < kNetCount; ++i)
+    if (std::string(kNetFields[i]) == name) return i;
+  return -1;
+}
 // Each kernel's parameters in order: what mxc_solver_call passes. An export listing another order is refused.
 struct KernelArguments {
   const char* kernel;
@@ -232,7 +240,7 @@ int read_export(const std::string& folder, mxc_solver* s, std::string& code, std
       return fail("mxc_solver_open: table " + t->name + " has " + std::to_string(bytes[i].size()) +
                   " bytes; its dtype and shape in meta.txt need " + std::to_string(t->bytes));
   }
-  const size_t atoms_bytes = bytes[kNetCount - 1].size();
+  const size_t atoms_bytes = bytes[net_field("atoms")].size();
   if (atoms_bytes != sizeof(double) * s->ns * s->nel)
     return fail("mxc_solver_open: the atoms table is not species x elements float64");
   std::string code_name;
@@ -534,7 +542,8 @@ int mxc_solver_atoms(const mxc_solver* s, double* atoms) {
   mxc_detail::clear_error();
   if (!s || !atoms) return fail("mxc_solver_atoms: null argument");
   if (check(hipSetDevice(s->device), "hipSetDevice")) return 1;
-  return check(hipMemcpy(atoms, s->tables[kNetCount - 1], sizeof(double) * s->ns * s->nel, hipMemcpyDeviceToHost),
+  return check(hipMemcpy(atoms, s->tables[net_field("atoms")], sizeof(double) * s->ns * s->nel,
+                         hipMemcpyDeviceToHost),
                "hipMemcpy");
 }
 

@@ -148,6 +148,7 @@ struct Net {
   const u32* leadw;
   const u32* chw;
   const double* atoms;  // [NS][NEL] network order
+  const int* rord;      // [NR] the order the rate coefficients are computed in (reactions grouped by law)
 };
 
 // ---- rate coefficients (the reference's law, collision_limit, gibbs_rt and rate_pair: common.cuh:135-242) ----
@@ -209,8 +210,10 @@ __device__ __forceinline__ void cell_rates(double* ws, double* rk, double T, dou
   for (int i = lane; i < NS; i += WAVE) g[i] = gibbs_rt(cf + 9 * i, T, lT, iT);
   if (lane == 0) g[NS] = 0.0;
   wave_sync();
+  const int* rord = net->rord;
 #pragma unroll 1
-  for (int r = lane; r < NR; r += WAVE) {
+  for (int q = lane; q < NR; q += WAVE) {
+    const int r = rord[q];  // a round's reactions mostly of one law, so it runs the code of few laws
     const int fl = rf[r];
     double f = 0.0, b = 0.0;
     if (fl & FL_PHOTO) {

@@ -22,17 +22,12 @@ def probe(net, photo, x, T, P, h, J):
     fn = mod.function("mxc_probe")
     ns = tab.ns
     perm = tab.perm
-    nasa_e = np.ascontiguousarray(tab.nasa[:, perm, :])
 
     def dev(a, dt=torch.float64):
         return torch.from_numpy(np.ascontiguousarray(a)).to(dtype=dt, device="cuda")
 
-    tabs = [dev(perm, torch.int32), dev(tab.rc), dev(tab.rf, torch.int32), dev(nasa_e),
-            dev(tab.rxe.view(np.int64), torch.int64), dev(tab.sv_words.ravel().view(np.int32), torch.int32),
-            dev(tab.row_pieces.view(np.int32), torch.int32), dev(tab.diag_words.ravel().view(np.int32), torch.int32),
-            dev(tab.lead_words.ravel().view(np.int32), torch.int32),
-            dev(np.concatenate([w.ravel() for w in tab.chunk_words]).view(np.int32), torch.int32),
-            dev(tab.atoms)]  # fmt: skip
+    host = _kernels.host_tables(tab)  # the solver's own tables, in the kernels' Net field order
+    tabs = [torch.from_numpy(host[k]).cuda() for k in _kernels.NET_FIELDS]
     net_struct = torch.tensor([a.data_ptr() for a in tabs], dtype=torch.int64, device="cuda")
     out = torch.zeros(8 + 8 * ns + 2 * ns * ns, dtype=torch.float64, device="cuda")
     xs = dev(x)

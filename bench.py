@@ -32,7 +32,7 @@ ROOT = Path(__file__).resolve().parent
 FP64_PEAK = 78.6  # TFLOP/s, MI355X FP64 vector
 # HBM bytes the plain FP64 call kernel moves per cell-call on the 69-species workload, from FETCH_SIZE and WRITE_SIZE
 # counters of the timed calls scaled by a 512 MiB copy (profiles/mxc_call_ncho_65536_traffic.md)
-TRAFFIC_PER_CELL_CALL = {"NCHO": 2026}
+TRAFFIC_PER_CELL_CALL = {"NCHO": 2011}
 
 
 def parse() -> argparse.Namespace:

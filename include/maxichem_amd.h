@@ -117,7 +117,7 @@ int mxc_solver_counts(const mxc_solver* solver, int* count, long long* total);
  * (cpp/maxichem.hpp:71). */
 int mxc_solver_atoms(const mxc_solver* solver, double* atoms);
 
-/* The parameter names mxc_solver_call passes to kernel (mxc_call, mxc_guard or mxc_recover), in order, separated by
+/* The parameter names mxc_solver_call passes to kernel (mxc_call or mxc_recover), in order, separated by
  * spaces, into names (size bytes with the NUL). An export must list the same. Replaces the reference caller's
  * kKernelArguments (cpp/maxichem.cpp:27-49). */
 int mxc_kernel_arguments(const char* kernel, char* names, size_t size);

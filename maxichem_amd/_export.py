@@ -20,7 +20,7 @@ from pathlib import Path
 from . import _constants as C
 from . import _hip, _kernels, _tables
 
-EXPORT_FORMAT = 2  # 2: the recovery workgroup's size (rec_threads)
+EXPORT_FORMAT = 3  # 2: the recovery workgroup's size (rec_threads); 3: the guard inside the call kernel
 CODE_FILE = "code.hsaco"
 _DTYPES = {"int32": "int32", "int64": "int64", "float64": "float64"}
 
@@ -54,7 +54,6 @@ def write(folder: str | Path, *, tab: _tables.Tables, source: str, code: bytes, 
         f"precision {precision}",
         f"lds_call {lds_call}",
         f"lds_rec {lds_rec}",
-        f"guard_threads {_kernels.GUARD_THREADS}",
         f"rec_threads {C.WAVE * _kernels.REC_WAVES}",
         f"rec_steps {C.REC_STEPS}",
         f"atol {C.ATOL!r}",

@@ -28,7 +28,7 @@ from ._tables import (
 )
 
 KERNEL_DIR = Path(__file__).resolve().parent / "kernels"
-KERNEL_FILES = ("common.hip", "call.hip", "guard.hip", "recover.hip")
+KERNEL_FILES = ("common.hip", "guard.hip", "call.hip", "recover.hip")
 ARCH = "gfx950"
 # hiprtc options: gfx950 code; packed FP32 instructions are kept out of all code (a known hazard on these parts)
 OPTIONS = (
@@ -41,15 +41,13 @@ OPTIONS = (
     "-Xclang",
     "-packed-fp32-ops",
 )
-GUARD_THREADS = 256
 REC_WAVES = 4  # wavefronts of a recovery workgroup: the first steps the cell, all of them share its dense LU
-KERNEL_NAMES = ("mxc_call", "mxc_guard", "mxc_recover")
+KERNEL_NAMES = ("mxc_call", "mxc_recover")
 # Each kernel's parameters in order (kernels/*.hip); an export lists them and the C++ loader refuses another list, as
 # the reference's KERNEL_ARGUMENTS (maxichem/_constants.py:148-164) does for its kernels.
 KERNEL_ARGUMENTS = {
-    "mxc_call": ("net", "x0", "Ta", "Pa", "dta", "hio", "xout", "doneo", "noko", "status", "count", "tries", "order",
-                 "Jin", "B", "rtol", "atol", "max_steps", "xprop", "rscratch", "fp64"),
-    "mxc_guard": ("net", "x0", "xout", "doneo", "noko", "status", "redo", "list", "count", "B", "atol"),
+    "mxc_call": ("net", "x0", "Ta", "Pa", "dta", "hio", "xout", "doneo", "noko", "status", "redo", "list", "count",
+                 "tries", "order", "Jin", "B", "rtol", "atol", "max_steps", "xprop", "rscratch", "fp64"),
     "mxc_recover": ("net", "x0", "Ta", "Pa", "dta", "hin", "xout", "hio", "doneo", "noko", "redo", "list", "count",
                     "tries", "Jin", "rtol", "atol", "max_steps", "rscratch"),
 }  # fmt: skip
@@ -83,7 +81,7 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixe
     ints = {
         "WAVE": C.WAVE, "NS": tab.ns, "NR": tab.nr, "NRT": tab.nrt, "NJ": tab.nj, "K0": tab.k0, "NT": tab.nt,
         "NEL": tab.nel, "RCW": C.RATE_ROW, "RROUNDS": -(-tab.nr // C.WAVE), "NDV": tab.ndv, "NPART": tab.npart,
-        "CPW": cpw, "STREAM_BATCH": batch, "GUARD_THREADS": GUARD_THREADS, "REC_WAVES": REC_WAVES, "CHUNK": CHUNK,
+        "CPW": cpw, "STREAM_BATCH": batch, "REC_WAVES": REC_WAVES, "CHUNK": CHUNK,
         "NCH": nch, "NLCH": tab.nlch,
         "SV_ROUNDS": tab.sv_words.shape[0], "DIAG_ROUNDS": tab.diag_words.shape[0],
         "LEAD_ROUNDS": tab.lead_words.shape[0], "REC_RETRIES": C.REC_RETRIES,
@@ -117,10 +115,11 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixe
 
 
 def lds_bytes(tab: Tables, cpw: int, mixed: bool = False, rates_global: bool = False) -> tuple[int, int]:
-    """Dynamic LDS of a call workgroup (cpw cells) and of a recovery workgroup (one cell, its dense W and its
-    factors' row order, and the word by which its first wavefront calls the others to a factorization)."""
+    """Dynamic LDS of a call workgroup (cpw cells) and of a recovery workgroup (one cell, its dense W, the
+    reciprocal pivots, a solve's vector, its factors' row order, and the word by which its first wavefront calls the
+    others to a factorization)."""
     size = tab.layout(mixed, not rates_global)["SIZE"]
-    return 8 * size * cpw, 8 * (size + tab.ns * tab.ns + 3 * tab.ns) + 4 * tab.ns + 4
+    return 8 * size * cpw, 8 * (size + tab.ns * tab.ns + 2 * tab.ns) + 4 * tab.ns + 4
 
 
 def host_tables(tab: Tables) -> dict[str, np.ndarray]:

@@ -1,6 +1,6 @@
 // A solver without Python: loads the folder a Python CudaSolver exported (code object, tables, meta.txt) and runs
-// the call kernel, the guard and the recovery on a HIP stream with the arguments the Python call passes, so a call
-// gives the same bits. The C ABI of include/maxichem_amd.h (mxc_solver_*); the reference's counterpart is the C++
+// the call kernel (which guards each cell's result) and the recovery on a HIP stream with the arguments the Python
+// call passes, so a call gives the same bits. The C ABI of include/maxichem_amd.h (mxc_solver_*); the reference's counterpart is the C++
 // caller cpp/maxichem.cpp:200-398, whose checks of the export come before any device work, as here.
 #include <hip/hip_runtime.h>
 
@@ -26,7 +26,7 @@ using mxc_detail::fail;
 
 namespace {
 
-constexpr int kExportFormat = 2;  // 2: the recovery workgroup's size (rec_threads)
+constexpr int kExportFormat = 3;  // 2: the recovery workgroup's size (rec_threads); 3: the guard in the call kernel
 constexpr int kWave = 64;  // lanes of a wavefront: one wavefront steps one cell
 // The kernels' Net struct, one device pointer per table in this order (kernels/common.hip).
 const char* const kNetFields[] = {"perm",  "rc",   "rf",  "nasa",  "rxe",   "svw",
@@ -45,9 +45,8 @@ struct KernelArguments {
   std::vector<std::string> names;
 };
 const KernelArguments kKernels[] = {
-    {"mxc_call", {"net", "x0", "Ta", "Pa", "dta", "hio", "xout", "doneo", "noko", "status", "count", "tries", "order",
-                  "Jin", "B", "rtol", "atol", "max_steps", "xprop", "rscratch", "fp64"}},
-    {"mxc_guard", {"net", "x0", "xout", "doneo", "noko", "status", "redo", "list", "count", "B", "atol"}},
+    {"mxc_call", {"net", "x0", "Ta", "Pa", "dta", "hio", "xout", "doneo", "noko", "status", "redo", "list", "count",
+                  "tries", "order", "Jin", "B", "rtol", "atol", "max_steps", "xprop", "rscratch", "fp64"}},
     {"mxc_recover", {"net", "x0", "Ta", "Pa", "dta", "hin", "xout", "hio", "doneo", "noko", "redo", "list", "count",
                      "tries", "Jin", "rtol", "atol", "max_steps", "rscratch"}},
 };
@@ -92,8 +91,8 @@ struct mxc_solver {
   int device = 0;
   std::string arch;  // the code object's architecture
   hipModule_t module = nullptr;
-  hipFunction_t call = nullptr, guard = nullptr, recover = nullptr;
-  int ns = 0, nj = 0, nr = 0, nel = 0, cpw = 1, lds_call = 0, lds_rec = 0, rates_global = 0, guard_threads = 0;
+  hipFunction_t call = nullptr, recover = nullptr;
+  int ns = 0, nj = 0, nr = 0, nel = 0, cpw = 1, lds_call = 0, lds_rec = 0, rates_global = 0;
   int rec_threads = 0;  // the recovery workgroup: wavefront 0 steps the cell, all of them share its dense LU
   int rec_steps = 0, max_steps = 0, max_blocks = 0, rec_blocks = 0;
   int bad_slot = -1, cold_slot = -1, hot_slot = -1;
@@ -180,12 +179,12 @@ int read_export(const std::string& folder, mxc_solver* s, std::string& code, std
   if (integer("ns", s->ns) || integer("nj", s->nj) || integer("nr", s->nr) || integer("nel", s->nel) ||
       integer("wave", wave) || integer("cpw", s->cpw) || integer("lds_call", s->lds_call) ||
       integer("lds_rec", s->lds_rec) || integer("rates_global", s->rates_global) ||
-      integer("guard_threads", s->guard_threads) || integer("rec_threads", s->rec_threads) ||
+      integer("rec_threads", s->rec_threads) ||
       integer("rec_steps", s->rec_steps) ||
       integer("max_steps", s->max_steps) || real("atol", s->atol) || real("rtol", s->rtol) ||
       real("t_min", s->t_min) || real("t_max", s->t_max) || real("x_sum_tol", s->x_sum_tol))
     return 1;
-  if (wave != kWave || s->ns < 1 || s->nj < 0 || s->nr < 1 || s->nel < 1 || s->cpw < 1 || s->guard_threads < 1 ||
+  if (wave != kWave || s->ns < 1 || s->nj < 0 || s->nr < 1 || s->nel < 1 || s->cpw < 1 ||
       s->rec_threads < kWave || s->rec_threads % kWave ||
       s->rec_steps < 1 || s->lds_call < 0 || s->lds_rec < 0)
     return fail("mxc_solver_open: meta.txt: sizes out of range");
@@ -381,7 +380,6 @@ int mxc_solver_open(const char* folder, int device, mxc_solver** solver) {
   const bool bad =
       check(hipModuleLoadData(&s->module, code.data()), "hipModuleLoadData") ||
       check(hipModuleGetFunction(&s->call, s->module, "mxc_call"), "kernel mxc_call") ||
-      check(hipModuleGetFunction(&s->guard, s->module, "mxc_guard"), "kernel mxc_guard") ||
       check(hipModuleGetFunction(&s->recover, s->module, "mxc_recover"), "kernel mxc_recover") ||
       check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "compute units") ||
       check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ_call, s->call, kWave * s->cpw, s->lds_call),
@@ -509,11 +507,9 @@ int mxc_solver_call(mxc_solver* s, void* stream_, int cells, const double* x, co
   };
   const unsigned call_grid = static_cast<unsigned>(std::max(1, std::min((cells + s->cpw - 1) / s->cpw, s->max_blocks)));
   if (launch(0, s->call, call_grid, kWave * s->cpw, s->lds_call)) return 1;
-  const unsigned guard_grid = static_cast<unsigned>(std::max(1, (cells + s->guard_threads - 1) / s->guard_threads));
-  if (launch(1, s->guard, guard_grid, s->guard_threads, 0)) return 1;
   v["max_steps"] = &rec_steps;
   const unsigned rec_grid = static_cast<unsigned>(std::max(1, std::min(cells, s->rec_blocks)));
-  if (launch(2, s->recover, rec_grid, s->rec_threads, s->lds_rec)) return 1;
+  if (launch(1, s->recover, rec_grid, s->rec_threads, s->lds_rec)) return 1;
   if (check(hipMemcpyAsync(s->last.data(), s->count, slots * sizeof(int), hipMemcpyDeviceToHost, stream),
             "copy counts") ||
       check(hipStreamSynchronize(stream), "hipStreamSynchronize"))  // the call's one host sync

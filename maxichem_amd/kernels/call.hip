@@ -1,5 +1,6 @@
 // The call kernel: a persistent grid of wavefronts, each stepping one cell at a time from an atomic queue through
-// its rates, Rodas3 tries and step control (the reference's call kernel, maxichem/kernels/call.cuh:5-178).
+// its rates, Rodas3 tries, step control and the guard of its result (the reference's call kernel,
+// maxichem/kernels/call.cuh:5-178, and its guard kernel, guard.cuh:114-140).
 
 // One try's four stages from the factors of M0 (call.cuh:119-158): K1 = W^-1 F, K2 = W^-1 (F + 4 K1 / h),
 // K3 = W^-1 (F(N + 2 K1) + (K1 - K2) / h), K4 = W^-1 (F(N + 2 K1 + K3) + (K1 - K2 - 8/3 K3) / h), with
@@ -90,10 +91,10 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_wa
     mxc_call(const Net* __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
              const double* __restrict__ Pa, const double* __restrict__ dta, double* __restrict__ hio,
              double* __restrict__ xout, int* __restrict__ doneo, double* __restrict__ noko,
-             int* __restrict__ status, int* __restrict__ count, double* __restrict__ tries,
-             const int* __restrict__ order, const double* __restrict__ Jin, int B, double rtol, double atol,
-             int max_steps, const double* __restrict__ xprop, double* __restrict__ rscratch,
-             int* __restrict__ fp64) {
+             int* __restrict__ status, int* __restrict__ redo, int* __restrict__ list, int* __restrict__ count,
+             double* __restrict__ tries, const int* __restrict__ order, const double* __restrict__ Jin, int B,
+             double rtol, double atol, int max_steps, const double* __restrict__ xprop,
+             double* __restrict__ rscratch, int* __restrict__ fp64) {
   extern __shared__ double lds[];
   const int lane = (int)(threadIdx.x & (WAVE - 1)), wv = (int)(threadIdx.x / WAVE);
   double* ws = lds + wv * L_SIZE;
@@ -120,7 +121,8 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_wa
     const double ps = wave_sum(p), pmax = wave_max(pm);
     const bool fine = fabs(ps - 1.0) <= XSUMTOL && pmax > 0.0 && isfinite(Tc) && isfinite(nd) && nd > 0.0 &&
                       isfinite(dt) && dt >= 0.0;
-    const int st = Tc < TMIN ? REFUSED_COLD : (Tc > TMAX ? REFUSED_HOT : (fine ? 0 : REFUSED_BAD_INPUT));
+    const int st = __builtin_amdgcn_readfirstlane(
+        Tc < TMIN ? REFUSED_COLD : (Tc > TMAX ? REFUSED_HOT : (fine ? 0 : REFUSED_BAD_INPUT)));
     if (lane == 0) status[cell] = st;
     if (st == 0) cell_rates(ws, rk, Tc, nd, net, Jc, lane);
     const double hin = hio[cell];
@@ -197,12 +199,30 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_wa
       it++;
       ntry++;
     }
-    write_state(Nl, Nt, xout + (size_t)cell * NS, net, lane);
+    // the result, and the guard while the state is in the lanes: a refused cell is counted by its status (and is
+    // not done, also at dt = 0); a stepped one whose result fails the guard is counted by what failed and listed
+    // for the recovery kernel
+    const bool fin = st == 0 && s.t >= dt * CALL_END;
+    double* xo = xout + (size_t)cell * NS;
+    int code = 0;
+    if (st == 0)
+      code = guard_wave(Nl, Nt, xr, xo, fin, s.nok, atol, net, lane);
+    else
+      write_state(Nl, Nt, xo, net, lane);
     if (lane == 0) {
       hio[cell] = s.h;
-      doneo[cell] = (st == 0 && s.t >= dt * CALL_END) ? 1 : 0;  // a refused cell is not done, also at dt = 0
+      doneo[cell] = fin ? 1 : 0;
       noko[cell] = s.nok;
       if (tries) tries[cell] = (double)ntry;
+      if (st != 0) {
+        atomicAdd(count + (st == REFUSED_COLD ? COUNT_COLD : st == REFUSED_HOT ? COUNT_HOT : COUNT_BAD_INPUT), 1);
+      } else if (code != 0) {
+        if (code & GUARD_NONFINITE) atomicAdd(count + COUNT_NONFINITE, 1);
+        if (code & GUARD_UNFINISHED) atomicAdd(count + COUNT_UNFINISHED, 1);
+        if (code & GUARD_DRIFT) atomicAdd(count + COUNT_DRIFT, 1);
+        list[atomicAdd(count + COUNT_RECOMPUTED, 1)] = cell;
+      }
+      redo[cell] = code != 0 ? 1 : 0;
     }
     int nx = 0;
     if (lane == 0) nx = waves + atomicAdd(count + COUNT_QUEUE, 1);

@@ -175,8 +175,7 @@ extern "C" __global__ void __launch_bounds__(WAVE* REC_WAVES)
   double* W = lds + L_SIZE;
   double* dinv = W + NS * NS;
   double* yv = dinv + NS;
-  double* xo = yv + NS;
-  int* pr = (int*)(xo + NS);
+  int* pr = (int*)(yv + NS);
   volatile int* cmd = pr + NS;
   if (wv != 0) {
     for (;;) {
@@ -259,18 +258,11 @@ extern "C" __global__ void __launch_bounds__(WAVE* REC_WAVES)
         it++;
         ntry++;
       }
-      // the result into LDS (network order), where the guard's test reads it
-      const double sum = vsum(Nl, Nt, lane);
-      if (lane < K0) xo[perm[lane]] = Nl / sum;
-      if (lane < NT) xo[perm[K0 + lane]] = Nt / sum;
-      wave_sync();
+      // the guard's test of the result, which goes to the cell's output row (a later pass overwrites it)
       const int dn = s.t >= dt * CALL_END ? 1 : 0;
-      int code = 0;
-      if (lane == 0) code = check_cell(xr, xo, dn, s.nok, opaque(net)->atoms, atol);
-      code = __builtin_amdgcn_readfirstlane(code);
-      wave_sync();
+      const int code = __builtin_amdgcn_readfirstlane(
+          guard_wave(Nl, Nt, xr, xout + (size_t)cell * NS, dn != 0, s.nok, atol, net, lane));
       if (code == GUARD_DRIFT && pass < REC_RETRIES) continue;  // only drifted: again, with a smaller first try
-      for (int i = lane; i < NS; i += WAVE) xout[(size_t)cell * NS + i] = xo[i];
       int nx = 0;
       if (lane == 0) {
         hio[cell] = s.h;

@@ -117,7 +117,6 @@ class CudaSolver(Solver):
         self._code = code = _hip.compile_source(self._source)
         self._module = _hip.Module(code, dev)
         self._call = self._module.function("mxc_call")
-        self._guard = self._module.function("mxc_guard")
         self._recover = self._module.function("mxc_recover")
         cus = info["compute_units"]
         self.max_blocks = cus * max(1, _hip.occupancy(self._call, WAVE * self.cpw, self.lds_call))
@@ -191,14 +190,17 @@ class CudaSolver(Solver):
         return order, None
 
     def _launch_all(self, x, T, P, dt, h, xo, ho, done, nok, status, redo, lst, jx, B, rtol, atol, max_steps,
-                    stream, order, xprop=None) -> None:  # fmt: skip
-        """The call, guard and recovery kernels on stream. ho holds each cell's first step size on entry (the
-        recovery starts from h); xprop, when given, the proposed end states [B, NS] (NaN first entry: none)."""
+                    stream, order, xprop=None, fp64=None) -> None:  # fmt: skip
+        """The call kernel (which guards each cell's result) and the recovery kernel on stream. ho holds each cell's
+        first step size on entry (the recovery starts from h); xprop, when given, the proposed end states [B, NS]
+        (NaN first entry: none); fp64, when given, the mixed form's per-cell FP64 flags to use instead of the
+        solver's own (a captured call owns its flags)."""
         p = _hip.ptr
         net = p(self._net)
-        call_args = [net, p(x), p(T), p(P), p(dt), p(ho), p(xo), p(done), p(nok), p(status), p(self.count),
-                     p(self.tries), p(order), p(jx), ctypes.c_int(B), ctypes.c_double(rtol), ctypes.c_double(atol),
-                     ctypes.c_int(max_steps), p(xprop), p(self._rscratch), p(self._fp64_cells(B))]  # fmt: skip
+        call_args = [net, p(x), p(T), p(P), p(dt), p(ho), p(xo), p(done), p(nok), p(status), p(redo), p(lst),
+                     p(self.count), p(self.tries), p(order), p(jx), ctypes.c_int(B), ctypes.c_double(rtol),
+                     ctypes.c_double(atol), ctypes.c_int(max_steps), p(xprop), p(self._rscratch),
+                     p(self._fp64_cells(B) if fp64 is None else fp64)]  # fmt: skip
         grid = max(1, min(-(-B // self.cpw), self.max_blocks))
         if self.timing is not None:
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -207,10 +209,6 @@ class CudaSolver(Solver):
         if self.timing is not None:
             ev[1].record()
             self.timing.append(ev)
-        guard_args = [net, p(x), p(xo), p(done), p(nok), p(status), p(redo), p(lst), p(self.count), ctypes.c_int(B),
-                      ctypes.c_double(atol)]  # fmt: skip
-        _hip.launch(self._guard, max(1, -(-B // _kernels.GUARD_THREADS)), _kernels.GUARD_THREADS, 0, stream,
-                    guard_args)  # fmt: skip
         rec_args = [net, p(x), p(T), p(P), p(dt), p(h), p(xo), p(ho), p(done), p(nok), p(redo), p(lst), p(self.count),
                     p(self.tries), p(jx), ctypes.c_double(rtol), ctypes.c_double(atol),
                     ctypes.c_int(REC_STEPS * max_steps), p(self._rscratch)]  # fmt: skip
@@ -269,8 +267,9 @@ class CudaSolver(Solver):
 
 class CallGraph:
     """One call of a solver captured in a HIP graph on fixed tensors (a GCM's time loop calls with the same tensors
-    every step): replay() runs the call kernel, guard and recovery again on what the input tensors hold then, with
-    one launch, and returns the graph's own output tensors, which the next replay overwrites. The counts, the
+    every step): replay() runs the call kernel (with its guard) and the recovery again on what the input tensors
+    hold then, with one launch, and returns the graph's own output tensors, which the next replay overwrites. The
+    graph owns every buffer its kernels address, the mixed form's FP64 flags included. The counts, the
     refusals and check() behave as in an eager call; the queue is ordered by the previous replay's tries, set on the
     host side before each replay. Building it runs the call once, off the graph, without adding to the counts."""
 
@@ -295,6 +294,8 @@ class CallGraph:
             self.nok = torch.empty(B, dtype=torch.float64, device=dev)
             self.tries = torch.zeros(B, dtype=torch.float64, device=dev)
             self.order = torch.arange(B, dtype=torch.int32, device=dev)
+            # the mixed form's per-cell FP64 flags, owned by the graph: its kernels hold their address
+            self.fp64 = torch.zeros(B, dtype=torch.int32, device=dev) if solver.precision == "mixed" else None
             self._ordered = False
             self.graph = torch.cuda.CUDAGraph()
             side = torch.cuda.Stream(dev)
@@ -316,7 +317,7 @@ class CallGraph:
         saved, s.tries = s.tries, self.tries
         stream = torch.cuda.current_stream(s.dev).cuda_stream
         s._launch_all(x, T, P, dt, h, self.xo, self.ho, self.done, self.nok, self.status, self.redo, self.lst,
-                      self.jx, self.B, rtol, atol, max_steps, stream, self.order)  # fmt: skip
+                      self.jx, self.B, rtol, atol, max_steps, stream, self.order, fp64=self.fp64)  # fmt: skip
         s.tries = saved
         if count:
             s.total += s.count

@@ -49,6 +49,9 @@ def lib() -> ctypes.CDLL:
         _lib.oracle_call.restype = ctypes.c_int
         _lib.oracle_constant.restype = ctypes.c_double
         _lib.oracle_constant.argtypes = [ctypes.c_char_p]
+        _lib.oracle_step_factor.restype = ctypes.c_double
+        _lib.oracle_step_factor.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_double]  # fmt: skip
     return _lib
 
 
@@ -122,6 +125,23 @@ class Oracle:
         )
         return F, JN
 
+    def try_loop(self, x0, T: float, P: float, dt: float, h0: float, *, rtol=1e-3, atol=1e-25, max_steps=4000,
+                 J=None, dense: bool = False) -> dict:  # fmt: skip
+        """One cell's tries without the guard and the recovery, as the call kernel takes them or (dense) as one
+        recovery pass does: the next step size h, the time reached t, the accepted steps and the tries."""
+        x0 = np.ascontiguousarray(x0, np.float64)
+        Jp = None
+        if self.tab.nj:
+            J = np.zeros(self.tab.nj) if J is None else np.ascontiguousarray(J, np.float64)
+            Jp = _p(J, _d)
+        h, t, nok, tries = (ctypes.c_double() for _ in range(4))
+        lib().oracle_try_loop(
+            ctypes.byref(self.net), _p(x0, _d), ctypes.c_double(T), ctypes.c_double(P), Jp, ctypes.c_double(dt),
+            ctypes.c_double(h0), ctypes.c_double(rtol), ctypes.c_double(atol), ctypes.c_int(max_steps),
+            ctypes.c_int(int(dense)), ctypes.byref(h), ctypes.byref(t), ctypes.byref(nok), ctypes.byref(tries),
+        )  # fmt: skip
+        return {"h": h.value, "t": t.value, "steps": nok.value, "tries": tries.value}
+
     def check_cell(self, x0, xout, done: bool, nok: float, atol: float = 1e-25) -> tuple[int, np.ndarray]:
         x0 = np.ascontiguousarray(x0, np.float64)
         xo = np.array(xout, np.float64)
@@ -142,6 +162,11 @@ def threshold_lu(a: np.ndarray, y: np.ndarray | None = None):
         ctypes.c_int(n), _p(a, _d), _p(pr, _i), _p(dinv, _d), ctypes.c_int(yy.shape[0]), _p(yy, _d)
     )
     return pr, a, dinv, yy
+
+
+def step_factor(en: float, ok: bool, prej: bool, pen: float, pha: float, ha: float) -> float:
+    """The oracle's step-size law: the factor on ha for the next try."""
+    return float(lib().oracle_step_factor(en, int(ok), int(prej), pen, pha, ha))
 
 
 def constant(name: str) -> float:

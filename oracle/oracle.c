@@ -675,6 +675,36 @@ int oracle_call(const oracle_net* net, int B, const double* x0, const double* T,
   return 0;
 }
 
+/* The step-size law alone (step_factor above), for the tests that pin it against known answers. */
+double oracle_step_factor(double en, int ok, int prej, double pen, double pha, double ha) {
+  return step_factor(en, ok, prej, pen, pha, ha);
+}
+
+/* The tries of one cell without the guard and the recovery: the call kernel's part of one_cell (dense 0), or one pass
+ * of the recovery (dense 1: the dense W with threshold pivoting), from x0 (network order) at h0 over dt, at most
+ * max_steps tries. Outputs the next step size, the time reached, the accepted steps and the tries. */
+void oracle_try_loop(const oracle_net* net, const double* x0, double T, double P, const double* J, double dt,
+                     double h0, double rtol, double atol, int max_steps, int dense, double* h, double* t, double* nok,
+                     double* tries) {
+  const int ns = net->ns;
+  net_column* col = net_columns(net);
+  cell_t c;
+  cell_alloc(&c, net, col);
+  c.nd = P / (KB * T);
+  oracle_rates(net, T, P, J, c.kf, c.kr);
+  int* inv = (int*)calloc((size_t)ns, sizeof(int));
+  for (int k = 0; k < ns; k++) inv[net->perm[k]] = k;
+  for (int i = 0; i < ns; i++) c.N[i] = x0[net->perm[i]];
+  *tries = 0.0;
+  const step_t s = try_loop(&c, inv, fmax(fmin(h0, dt), H_MIN), dt, max_steps, rtol, atol, dense, tries);
+  *h = s.h;
+  *t = s.t;
+  *nok = s.nok;
+  free(inv);
+  free(col);
+  cell_free(&c);
+}
+
 /* F(N) and the full Jacobian dF/dN = Jn + w 1^T at N (network order): the oracle's own pieces, for the tests that
  * check them against finite differences and dense solves. */
 void oracle_rhs_jacobian(const oracle_net* net, double T, double P, const double* J, const double* N, double* F,

@@ -49,6 +49,17 @@ void oracle_rhs_jacobian(const oracle_net* net, double T, double P, const double
  * y [m][n]. */
 void oracle_threshold_lu(int n, double* A, int* pr, double* dinv, int m, double* y);
 
+/* The step-size law: the factor on the try's step ha for the next try, from the try's error en, whether it was
+ * accepted, and the previous try's rejection, error and step (step_factor of common.cuh:87-100). */
+double oracle_step_factor(double en, int ok, int prej, double pen, double pha, double ha);
+
+/* One cell's tries without the guard and the recovery, as the call kernel takes them (dense 0) or as one recovery
+ * pass does (dense 1): the next step size h, the time reached t, the accepted steps nok and the tries. x0 in network
+ * order, J [nj] or NULL. */
+void oracle_try_loop(const oracle_net* net, const double* x0, double T, double P, const double* J, double dt,
+                     double h0, double rtol, double atol, int max_steps, int dense, double* h, double* t, double* nok,
+                     double* tries);
+
 /* The guard's test and share correction of one result (check_cell): returns the GUARD_* bits, 0 when it passes, in
  * which case xout is corrected in place. */
 int oracle_check_cell(const oracle_net* net, const double* x0, double* xout, int done, double nok, double atol);

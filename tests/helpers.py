@@ -86,6 +86,20 @@ def state(net, cells: int, **fractions) -> np.ndarray:
     return x / x.sum(1, keepdims=True)
 
 
+def adversarial_matrices(n: int, rng: np.random.Generator, count: int = 8) -> np.ndarray:
+    """Matrices for the recovery's threshold LU: count diagonally dominant ones with their rows shuffled (rows exchange
+    at most pivots), count random ones with every other diagonal scaled by 1e-3 (some rows kept by the threshold
+    where partial pivoting would exchange), and the first random one with column 0's largest |entry| twice, in rows
+    3 and 7 (the tie goes to the lower row)."""
+    shuffled = 0.1 * rng.standard_normal((count, n, n)) + 10.0 * np.eye(n)
+    shuffled = np.stack([q[rng.permutation(n)] for q in shuffled])
+    random = rng.standard_normal((count, n, n))
+    random[:, range(0, n, 2), range(0, n, 2)] *= 1e-3
+    tie = random[0].copy()
+    tie[[3, 7], 0] = (10.0, -10.0)
+    return np.concatenate([shuffled, random, tie[None]])
+
+
 def threshold_lu_numpy(a: np.ndarray, pivtau: float = 0.1):
     """The recovery's rule written out in numpy: at pivot k take the first row of the largest |entry| at or below k,
     and exchange only when |a_kk| < pivtau times it. Returns the row order, the factors, exchanges and kept rows."""
@@ -141,7 +155,8 @@ class Backend:
         r = self.impl.launch(t(x), t(T), t(P), t(dt), t(h), rtol=rtol, atol=atol, max_steps=max_steps, J=Jt)
         counts = self.impl.count.cpu().numpy().astype(np.int64)
         return {"x": r.x.cpu().numpy(), "h": r.h.cpu().numpy(), "done": r.done.cpu().numpy(),
-                "steps": r.steps.cpu().numpy(), "flag": r.flag.cpu().numpy(), "count": counts}  # fmt: skip
+                "steps": r.steps.cpu().numpy(), "flag": r.flag.cpu().numpy(), "count": counts,
+                "tries": self.impl.tries.cpu().numpy()}  # fmt: skip
 
     def close(self):
         if self.kind != "oracle":

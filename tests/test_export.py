@@ -2,9 +2,10 @@
 
 - without a GPU: the export's files and meta.txt, the kernel argument lists (the kernels' own parameters, the Python
   list and the C ABI's list agree), and mxc_solver_open refusing a bad export before any device work;
-- on the GPU: the C ABI's solver (through ctypes) and the C++ class maxichem::Solver (tests/cpp/test_export.cpp,
-  a program linked against libtorch) against the Python call, bit for bit, counts and flags included, for thermal
-  and photolysis networks, a forced recovery, the rate coefficients in global scratch, and the refusals.
+- on the GPU: the C ABI's solver (through ctypes) and the C++ class maxichem::Solver (driven by
+  tests/cpp/cpp_caller.cpp, a program linked against libtorch that runs a plan of calls and writes what each
+  returned) against the Python call, bit for bit, counts and flags included, for thermal and photolysis networks, a
+  forced recovery, the rate coefficients in global scratch, and the refusals.
 """
 
 import ctypes
@@ -20,7 +21,7 @@ from maxichem_amd import _export, _hip, _kernels, standin, workloads
 from maxichem_amd._constants import COUNT_SLOTS, DEFAULT_MAX_STEPS
 
 ROOT = Path(__file__).resolve().parent.parent
-PROGRAM = ROOT / "tests" / "cpp" / "test_export"
+PROGRAM = ROOT / "tests" / "cpp" / "cpp_caller"
 CELLS = 2048
 FORCED_TRIES = 2  # most cells end unfinished and the recovery recomputes them
 
@@ -84,14 +85,14 @@ def open_error(folder: Path) -> str:
 
 
 EDITS = {  # meta.txt edits mxc_solver_open refuses before any device work, and the message each gives
-    "format": ("export_format 2", "export_format 3", "export format 3, this library reads 2"),
+    "format": ("export_format 3", "export_format 2", "export format 2, this library reads 3"),
     "no recovery workgroup": ("rec_threads 256", "rec_threads 96", "sizes out of range"),
     "unknown": ("args mxc_recover net", "args mxc_recover net2", "kernel mxc_recover takes argument net2, which"),
-    "repeated": ("args mxc_guard net x0 xout", "args mxc_guard net x0 x0 xout", "kernel mxc_guard lists argument x0 twice"),
+    "repeated": ("args mxc_recover net x0 Ta", "args mxc_recover net x0 x0 Ta",
+                 "kernel mxc_recover lists argument x0 twice"),
     "out of order": ("args mxc_call net x0 Ta Pa", "args mxc_call net x0 Pa Ta",
                      "kernel mxc_call has argument Pa at position 2, where this library passes Ta"),
-    "dropped": ("args mxc_guard net x0 xout doneo noko status redo list count B atol",
-                "args mxc_guard net x0 xout doneo noko status redo list count B", "kernel mxc_guard lacks argument atol"),
+    "dropped": ("max_steps rscratch\n", "max_steps\n", "kernel mxc_recover lacks argument rscratch"),
     "net order": ("net perm rc rf", "net rc perm rf", "net field 0 is rc, the kernels read perm there"),
     "table shape": ("table atoms float64 69 5", "table atoms float64 69 6", "table atoms has 2760 bytes"),
 }  # fmt: skip
@@ -126,8 +127,8 @@ def save(folder: Path, **arrays) -> None:
 
 
 def make_case(solver, folder: Path, cells: int, seed: int, max_steps: int, edit: str | None = None) -> dict:
-    """Inputs of the first call from cold (solar gas, dt = 300 s, J on the lowest-P quarter), the Python call's
-    results or refusal, written for the C++ program."""
+    """Inputs of the first call from cold (solar gas, dt = 300 s, J on the lowest-P quarter), written for the C++
+    driver, and the Python call's results or refusal."""
     net_species = solver._names[0]
     d = workloads.config1(net_species, max(solver.nj, 1), cells=cells, seed=seed)
     x, T, P, dt, h = d["x"], d["T"], d["P"], d["dt"], d["h"]
@@ -148,26 +149,39 @@ def make_case(solver, folder: Path, cells: int, seed: int, max_steps: int, edit:
     folder.mkdir(parents=True, exist_ok=True)
     save(folder, x=x, T=T, P=P, dt=dt, h=h, **({"J": J} if solver.nj else {}))
     (folder / "B.txt").write_text(f"{cells}\n")
-    (folder / "max_steps.txt").write_text(f"{max_steps}\n")
     inputs = dict(x=g(x), T=g(T), P=g(P), dt=g(dt), h=g(h), J=g(J) if solver.nj else None)
     try:
         r = solver.launch(inputs["x"], inputs["T"], inputs["P"], inputs["dt"], inputs["h"], max_steps=max_steps,
                           J=inputs["J"])  # fmt: skip
     except ValueError as e:
-        (folder / "refused.txt").write_text(str(e))
-        save(folder, counts_out=solver.count.cpu().numpy())
-        return {"inputs": inputs, "refused": str(e)}
-    save(folder, x_out=r.x.cpu().numpy(), h_out=r.h.cpu().numpy(), steps_out=r.steps.cpu().numpy(),
-         done_out=r.done.cpu().numpy().astype(np.int8), counts_out=solver.count.cpu().numpy())  # fmt: skip
+        return {"inputs": inputs, "refused": str(e), "count": solver.count.cpu().numpy()}
     return {"inputs": inputs, "result": r, "count": solver.count.cpu().numpy()}
 
 
-def run_program(export: Path, case: Path) -> subprocess.CompletedProcess:
+def run_program(export: Path, case: Path, plan: list[str]) -> tuple[subprocess.CompletedProcess, Path]:
+    """Runs the C++ driver on a plan of steps (tests/cpp/cpp_caller.cpp); returns its process and output folder."""
     if not PROGRAM.exists():
         raise FileNotFoundError(f"{PROGRAM} is missing: build it (python -c 'import __graft_entry__ as g; g.build()')")
-    r = subprocess.run([str(PROGRAM), str(export), str(case)], capture_output=True, text=True, timeout=300)
+    (case / "plan.txt").write_text("\n".join(plan) + "\n")
+    out = case / "out"
+    out.mkdir(exist_ok=True)
+    r = subprocess.run([str(PROGRAM), str(export), str(case), str(out)], capture_output=True, text=True, timeout=300)
     print(r.stdout, r.stderr[-2000:], flush=True)
-    return r
+    return r, out
+
+
+def program_call(out: Path, tag: str, cells: int, ns: int) -> dict:
+    """What one call of the driver's plan returned: the outputs and counts, or the exception's kind and message."""
+    res = {"count": np.fromfile(out / f"{tag}.count.bin", np.int32)}
+    err = out / f"{tag}.error.txt"
+    if err.exists():
+        res["kind"], res["message"] = err.read_text().split("\n", 1)
+        return res
+    res["x"] = np.fromfile(out / f"{tag}.x.bin").reshape(cells, ns)
+    for name in ("h", "steps"):
+        res[name] = np.fromfile(out / f"{tag}.{name}.bin")
+    res["done"] = np.fromfile(out / f"{tag}.done.bin", np.int8).astype(bool)
+    return res
 
 
 class AbiSolver:
@@ -245,11 +259,33 @@ def test_abi_and_cpp_calls_are_the_python_call(gpu, tmp_path, which, max_steps):
             assert np.array_equal(c, count) and np.array_equal(t, (k + 1) * count.astype(np.int64))
     finally:
         a.close()
-    r = run_program(tmp_path / "export", tmp_path / "case")
-    assert r.returncode == 0 and "bitwise identical to the Python call" in r.stdout
-    assert ("(default arguments)" in r.stdout) == (max_steps == DEFAULT_MAX_STEPS)
-    assert ("J [B, 0] gives the same bits, J [B, 1] refused" in r.stdout) == (not photo)
-    assert "HIP runtimes mapped 1" in r.stdout
+    # the C++ class: two calls (the work buffers reused), a thermal network's empty and refused J, then the
+    # accessors' tensors changed in place before a last call
+    steps = "default" if max_steps == DEFAULT_MAX_STEPS else str(max_steps)
+    j = "given" if photo else "none"
+    plan = [f"call first {steps} {j}", f"call second {steps} {j}"]
+    plan += [] if photo else [f"call empty {steps} empty", f"call columns {steps} columns"]
+    plan += ["state before", "scramble", f"call last {steps} {j}", "state after"]
+    r, out = run_program(tmp_path / "export", tmp_path / "case", plan)
+    assert r.returncode == 0
+    ns = want.x.shape[1]
+    tags = ["first", "second", "last"] + ([] if photo else ["empty"])
+    for tag in tags:
+        got = program_call(out, tag, CELLS, ns)
+        for name in ("x", "h", "steps"):
+            assert np.array_equal(got[name].view(np.uint8), bits(getattr(want, name))), (tag, name)
+        assert np.array_equal(got["done"], want.done.cpu().numpy()) and np.array_equal(got["count"], count), tag
+    if not photo:
+        got = program_call(out, "columns", CELLS, ns)
+        assert got["kind"] == "c10" and "J: none or an empty" in got["message"]
+    before, after = ({k: np.fromfile(out / f"{tag}.{k}.bin", dt) for k, dt in
+                      (("count", np.int32), ("total", np.int64), ("atoms", np.float64))} for tag in ("before", "after"))  # fmt: skip
+    calls = len(tags)
+    assert np.array_equal(before["total"], (calls - 1) * count.astype(np.int64))
+    assert np.array_equal(after["total"], calls * count.astype(np.int64))  # the solver's own totals, not the copy
+    assert np.array_equal(after["atoms"], before["atoms"]) and np.array_equal(after["count"], count)
+    assert np.array_equal(after["atoms"].reshape(ns, -1), standin.load(shape).atoms)
+    assert (out / "runtimes.txt").read_text() == "1"  # the library shares torch's HIP runtime
 
 
 @pytest.mark.gpu
@@ -270,9 +306,14 @@ def test_cpp_refuses_what_python_refuses(gpu, tmp_path, edit):
             assert rc == 2 and a.err() == case["refused"]
         finally:
             a.close()
-    r = run_program(tmp_path / "export", tmp_path / "case")
+    r, out = run_program(tmp_path / "export", tmp_path / "case", ["call first default given"])
     assert r.returncode == 0
-    assert ("refused as the Python call" if refused else "bitwise identical") in r.stdout
+    got = program_call(out, "first", 256, 69)
+    if refused:
+        assert got["kind"] == "invalid_argument" and got["message"] == "maxichem: " + case["refused"]
+        assert np.array_equal(got["count"], case["count"])
+    else:
+        assert np.array_equal(got["x"].view(np.uint8), bits(case["result"].x))
 
 
 @pytest.mark.gpu
@@ -280,8 +321,8 @@ def test_cpp_constructor_refuses_another_contract(gpu, ncho_export, tmp_path):
     folder = tmp_path / "export"
     subprocess.run(["cp", "-r", str(ncho_export), str(folder)], check=True)
     text = (folder / "meta.txt").read_text()
-    (folder / "meta.txt").write_text(text.replace("args mxc_guard net x0 xout", "args mxc_guard net x0 x0 xout"))
+    (folder / "meta.txt").write_text(text.replace("args mxc_recover net x0 Ta", "args mxc_recover net x0 x0 Ta"))
     (tmp_path / "case").mkdir()
-    r = run_program(folder, tmp_path / "case")
-    assert r.returncode != 0 and "kernel mxc_guard lists argument x0 twice" in r.stderr
-    assert "export loaded" not in r.stdout
+    r, _ = run_program(folder, tmp_path / "case", [])
+    assert r.returncode == 3 and "kernel mxc_recover lists argument x0 twice" in r.stderr
+    assert "opened" not in r.stdout

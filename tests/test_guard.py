@@ -289,8 +289,11 @@ ATOMS = {"H": {"H": 1}, "H2": {"H": 2}, "OH": {"H": 1, "O": 1}, "H2O": {"H": 2, 
 DRIFTS = (0.5, 0.99, 1.01, 2.0)  # the largest relative change of an element share, in allowances
 
 
-def test_guard_kernel_flags_drift_beyond_the_allowance(read_network, gpu):
-    # the guard kernel alone, on states whose He share moved by r allowances; the oracle's check_cell agrees
+def test_guard_flags_drift_beyond_the_allowance(read_network, gpu):
+    # the call kernel's guard alone (tests/recovery_probe.hip), on states whose He share moved by r allowances; the
+    # oracle's check_cell agrees
+    from probes import Probes
+
     net = read_network("synthetic_small")
     x0 = np.array([{"H": 0.1, "H2": 0.6, "OH": 0.05, "H2O": 0.1, "O": 0.05, "He": 0.1}[q] for q in net.species])
     atoms = np.array([[ATOMS[q].get(e, 0) for e in ("H", "O", "He")] for q in net.species], dtype=float)
@@ -303,25 +306,54 @@ def test_guard_kernel_flags_drift_beyond_the_allowance(read_network, gpu):
         x[he] *= 1.0 + d
         moved = (x @ atoms) / (x @ atoms).sum() / share - 1.0
         assert np.abs(moved).max() == pytest.approx(r * ALLOWANCE, rel=1e-6)
-        results.append(x)
-    cells = len(DRIFTS)
-    with CudaSolver(net, gpu, drift=ALLOWANCE) as solver:
-        x0d, xo = t([x0] * cells), t(results)
-        done = torch.ones(cells, dtype=torch.int32, device="cuda")
-        nok = full(cells, 1.0)
-        status, redo, lst = (torch.zeros(cells, dtype=torch.int32, device="cuda") for _ in range(3))
-        solver.count.zero_()
-        p = _hip.ptr
-        args = [p(solver._net), p(x0d), p(xo), p(done), p(nok), p(status), p(redo), p(lst), p(solver.count),
-                ctypes.c_int(cells), ctypes.c_double(1e-25)]  # fmt: skip
-        _hip.launch(solver._guard, 1, _kernels.GUARD_THREADS, 0, torch.cuda.current_stream().cuda_stream, args)
-        counts = solver.fallback
+        results.append(x / x.sum())
+    probes = Probes(net, drift=ALLOWANCE)
+    try:
+        _, code = probes.run_guard(np.array([x0] * len(DRIFTS)), np.array(results), 1, 1.0)
+    finally:
+        probes.close()
     orc = oracle.Oracle(_tables.build(net, False), ALLOWANCE)
-    want = [int(orc.check_cell(x0, x, True, 1.0)[0] != 0) for x in results]
-    print(f"drift {DRIFTS} allowances: flagged {redo.tolist()}, oracle {want}; counts {counts}")
-    assert redo.tolist() == [int(r > 1.0) for r in DRIFTS] == want
-    assert counts["drift"] == counts["recomputed"] == sum(r > 1.0 for r in DRIFTS)
+    want = [orc.check_cell(x0, x, True, 1.0)[0] for x in results]
+    print(f"drift {DRIFTS} allowances: guard bits {code.tolist()}, oracle {want}")
+    assert code.tolist() == want == [16 * int(r > 1.0) for r in DRIFTS]
 
+
+GUARD_CASES = 64
+
+
+def test_guard_matches_the_oracle(gpu):
+    # the guard on NCHO results (finished and not, drifted, non-finite) against the oracle's check_cell: the same
+    # bits, and the corrected shares within rounding (the sums run in another order); the lanes' sums of the
+    # correction restore the input's element shares to 1e-14
+    from probes import Probes
+
+    net = standin.load("NCHO")
+    tab = _tables.build(net, False)
+    rng = np.random.default_rng(3)
+    inp = workloads.config0(net.species, 0, cells=GUARD_CASES, seed=4)
+    x0 = inp["x"]
+    res = x0 * np.exp(rng.normal(0.0, 1e-7, x0.shape))  # small moves: corrected
+    res[8:16] = x0[8:16] * np.exp(rng.normal(0.0, 1e-2, (8, tab.ns)))  # large moves: drift
+    res[16, 5] = np.inf
+    res[17, 0] = np.nan
+    done = np.ones(GUARD_CASES, np.int32)
+    done[20:24] = 0
+    nok = rng.integers(1, 50, GUARD_CASES).astype(np.float64)
+    probes = Probes(net)
+    try:
+        out, code = probes.run_guard(x0, res, done, nok)
+    finally:
+        probes.close()
+    orc = oracle.Oracle(tab)
+    atoms = tab.atoms
+    for b in range(GUARD_CASES):
+        want_code, want = orc.check_cell(x0[b], res[b] / res[b].sum(), bool(done[b]), nok[b])
+        assert code[b] == want_code, b
+        if want_code == 0:
+            np.testing.assert_allclose(out[b], want, rtol=1e-13, atol=0)
+            s0, s1 = x0[b] @ atoms, out[b] @ atoms
+            assert np.abs(s1 / s1.sum() - s0 / s0.sum()).max() <= 1e-14 * (s0 / s0.sum()).max()
+    assert sorted(set(code.tolist())) == [0, 4, 8, 16]
 
 
 def kernel_sum(row, perm, k0, nt):

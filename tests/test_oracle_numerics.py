@@ -21,20 +21,10 @@ def test_oracle_constants_are_the_packages():
         assert oracle.constant(name) == float(getattr(_constants, name)), name
 
 
-def _matrices(n, rng, count=8):
-    shuffled = 0.1 * rng.standard_normal((count, n, n)) + 10.0 * np.eye(n)
-    shuffled = np.stack([q[rng.permutation(n)] for q in shuffled])
-    random = rng.standard_normal((count, n, n))
-    random[:, range(0, n, 2), range(0, n, 2)] *= 1e-3
-    tie = random[0].copy()
-    tie[[3, 7], 0] = (10.0, -10.0)
-    return np.concatenate([shuffled, random, tie[None]])
-
-
 def test_threshold_lu_follows_the_rule():
     rng = np.random.default_rng(11)
     n = 42
-    mats = _matrices(n, rng)
+    mats = helpers.adversarial_matrices(n, rng)
     exchanged, kept_rows = [], []
     for m, a in enumerate(mats):
         y = rng.standard_normal(n)

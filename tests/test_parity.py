@@ -17,6 +17,9 @@ pytestmark = pytest.mark.gpu
 REL = 1e-2
 FLOOR = 1e-20
 SHARE_DRIFT = 1e-10
+STEPS_SAME = 0.95  # share of cells whose accepted steps and tries are the oracle's
+STEPS_REL = 0.05  # the largest relative difference of accepted steps on the others
+H_SAME = 0.95  # share of those cells whose next step size is the oracle's to REL
 
 
 def t(a):
@@ -36,17 +39,29 @@ def compare(name, got, want, x0, atoms):
     drift = np.abs(s1 - s0) / np.maximum(s0, 1e-300)
     drift = np.where(s0 > 1e-12, drift, 0.0)
     steps = np.abs(got["steps"] - want["steps"]) / np.maximum(want["steps"], 1.0)
+    same = (got["steps"] == want["steps"]) & (got["tries"] == want["tries"])  # the same tries: the same sequence
+    hrel = np.abs(got["h"][same] / want["h"][same] - 1.0)
     print(f"{name}: max rel {rel.max():.2e}, p99 {np.percentile(rel, 99):.2e}; share drift {drift.max():.1e}; "
-          f"done {int(got['done'].sum())}/{int(want['done'].sum())}; steps rel diff max {steps.max():.2e}")  # fmt: skip
+          f"done {int(got['done'].sum())}/{int(want['done'].sum())}; steps and tries equal on "
+          f"{int(same.sum())}/{same.size}, steps rel diff max {steps.max():.2e}; h rel max on those "
+          f"{hrel.max() if hrel.size else 0.0:.2e}")  # fmt: skip
     assert np.array_equal(got["done"], want["done"])
     assert rel.max() <= REL
     assert drift.max() <= SHARE_DRIFT
+    # the step sequence: the same accepted steps and tries on all but rare cells where rounding tips one accept or
+    # reject, and then the same next step size (the caller carries h into the next call) on all but a tail: the
+    # last try's error estimate is rounding-dominated where trace species sit near atol (K4 of a species at 1e-20
+    # carries rounding of its coupled species' terms), and h follows it as err^(-1/3)
+    assert same.mean() >= STEPS_SAME and steps.max() <= STEPS_REL
+    assert hrel.size == 0 or (hrel <= REL).mean() >= H_SAME
+    if "flag" in got and "flag" in want:
+        assert np.array_equal(got["flag"], want["flag"])
 
 
 def run_gpu(solver, inp, dt_key="dt", J=True):
     r = solver.launch(t(inp["x"]), t(inp["T"]), t(inp["P"]), t(inp[dt_key]), t(inp["h"]),
                       J=t(inp["J"]) if J else None)  # fmt: skip
-    return {k: v.cpu().numpy() for k, v in r._asdict().items()}
+    return {**{k: v.cpu().numpy() for k, v in r._asdict().items()}, "tries": solver.tries.cpu().numpy()}
 
 
 @pytest.mark.parametrize("shape", ["NCHO", "SNCHO"])
@@ -95,7 +110,7 @@ def test_synthetic_networks_match_the_oracle(read_network, stem):
     want = oracle.Oracle(_tables.build(net, photo)).call(x, T, P, dt, h, J=J)
     with CudaSolver(net, "cuda", photo=photo) as solver:
         r = solver.launch(t(x), t(T), t(P), t(dt), t(h), J=None if J is None else t(J))
-        got = {k: v.cpu().numpy() for k, v in r._asdict().items()}
+        got = {**{k: v.cpu().numpy() for k, v in r._asdict().items()}, "tries": solver.tries.cpu().numpy()}
     compare(stem, got, want, x, net.atoms)
 
 

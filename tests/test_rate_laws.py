@@ -1,231 +1,236 @@
-"""Rate coefficients of every law against the laws written out here, on the oracle and the GPU.
+"""Every rate law the reader takes, against closed forms evaluated here from the cases of
+tests/golden/data/rate_laws.json (the reference checks the same laws in its tests/test_rate_laws.py and
+tests/test_collision_limit.py).
 
-Each reaction turns one isomer into another beside spectators, so the reactant relaxes to the pair's equilibrium at
-lambda = c (kf + kr) times the spectators' densities, exactly: x(t) = x_eq + (x0 - x_eq) exp(-lambda t), with c the
-collision cap. A cell per reaction runs dt = 1 / lambda; the lambda seen is the solver's rate coefficient. The cases
-follow the reference's tests/test_rate_laws.py and tests/test_collision_limit.py; the closed forms are written here.
+Three checks:
+- the oracle's rate coefficients (forward and reverse, the collision cap applied) against the closed forms, directly;
+- the call on the oracle and on the GPU: each case is an isomerisation A <-> B beside spectators, so A relaxes to the
+  pair's equilibrium at lambda = cap (kf + kr) times the spectators' densities, and after dt = 1 / lambda the
+  distance to equilibrium has shrunk by exactly e;
+- falloff rows at the ends of Pr, and the collision cap's closed forms on the reference's collision network.
 """
 
-from dataclasses import dataclass, field
+import json
+import math
+from pathlib import Path
 
 import helpers
 import numpy as np
 import pytest
 from helpers import BACKENDS
 
-K_COLL = 1e-9
-TROE_C, TROE_N, TROE_D = (-0.4, -0.67), (0.75, -1.27), 0.14
-CONDITIONS = [(600.0, 1e4), (1500.0, 1e6), (2600.0, 1e8)]
-RTOL = 1e-9
+CASES = json.loads((Path(__file__).resolve().parent / "golden" / "data" / "rate_laws.json").read_text())
+K_COLL, TRACE = CASES["k_coll"], CASES["trace"]
+CONDITIONS = [tuple(c) for c in CASES["conditions"]]
+RTOL = 1e-9  # the call's tolerance: its integration error on lambda dt = 1 stays far below RATE_TOLERANCE
 RATE_TOLERANCE = 1e-6
-TRACE = 1e-9
-BOUND = -2.0e5
+COEFFICIENT_TOLERANCE = 1e-12  # the oracle's coefficients against the closed forms: rounding of exp and log
+# VULCAN's section headings, by the case files' section keys
+SECTION_HEADINGS = {"two": "# Two-body Reactions", "three": "# 3-body and Disscoiation Reactions",
+                    "low": "# 3-body reactions without high-pressure rates", "special": "# special cases"}  # fmt: skip
 
 
-def arrhenius(a, b, c, T):
-    return float(a * T**b * np.exp(-c / T))
+# ---- the laws, written out (Troe 1983 and Venot et al. 2012 eq. C.2-C.5; Visscher & Moses 2011) ----
 
 
-def troe_factor(T, pr, troe, vm11):
+def modified_arrhenius(term, T):
+    A, B, C = term
+    return A * T**B * math.exp(-C / T)
+
+
+def log10_fcent(troe, T):
     a, t3, t1, t2 = troe
-    lfc = np.log10((1.0 - a) * np.exp(-T / t3) + a * np.exp(-T / t1) + np.exp(-t2 / T))
-    n = TROE_N[0] + TROE_N[1] * lfc
+    return math.log10((1.0 - a) * math.exp(-T / t3) + a * math.exp(-T / t1) + math.exp(-t2 / T))
+
+
+def broadening(troe, vm11, T, pr):
+    """F of the falloff k = F k0 / (1 + Pr): log10 F = log10 Fc / (1 + f^2), f from log10 Pr."""
+    lfc = log10_fcent(troe, T)
+    width = 0.75 - 1.27 * lfc
     if vm11:
-        f = np.log10(pr) / n
+        f = math.log10(pr) / width
     else:
-        shifted = np.log10(pr) + TROE_C[0] + TROE_C[1] * lfc
-        f = shifted / (n - TROE_D * shifted)
-    return float(10.0 ** (lfc / (1.0 + f * f)))
+        z = math.log10(pr) - 0.4 - 0.67 * lfc
+        f = z / (width - 0.14 * z)
+    return 10.0 ** (lfc / (1.0 + f * f))
 
 
-def troe_limit(T, troe):
-    """F at Pr -> 0: log10 F = log10 Fc / (1 + 1 / d^2)."""
-    a, t3, t1, t2 = troe
-    fc = (1.0 - a) * np.exp(-T / t3) + a * np.exp(-T / t1) + np.exp(-t2 / T)
-    return float(fc ** (TROE_D**2 / (1.0 + TROE_D**2)))
+def broadening_at_low_pressure(troe, T):
+    """F as Pr -> 0 in Troe's form: f -> -1 / 0.14, so F = Fc^(0.14^2 / (1 + 0.14^2))."""
+    return 10.0 ** (log10_fcent(troe, T) * 0.14**2 / (1.0 + 0.14**2))
 
 
-def falloff(k0, kinf, nd, T, troe, vm11=False):
-    pr = k0 * nd / kinf
-    return k0 / (1.0 + pr) * (troe_factor(T, pr, troe, vm11) if troe else 1.0)
+def coefficient(law, T, nd):
+    """k of a case's law at T and total density nd, before a third body's nd is folded in."""
+    k = sum(modified_arrhenius(term, T) for term in law["low"])
+    if law.get("high") is not None:
+        pr = k * nd / modified_arrhenius(law["high"], T)
+        k = k / (1.0 + pr) * (broadening(law["troe"], law.get("vm11", False), T, pr) if law.get("troe") else 1.0)
+    return k * law.get("eff", 1.0)
 
 
-FALLOFF_K0, FALLOFF_KINF = (3.0e-14, -1.0, 0.0), (3.16, 0.5, 0.0)
-TROE = (0.6, 100.0, 2000.0, 5000.0)
-LOW = (3.0e-13, -0.5, 800.0)
-SPECIAL_K0 = ((1.932e3, -9.88, 7544.0), (5.109e-11, -6.25, 1433.0))
-SPECIAL_KINF = (1.031e-10, -0.018, -16.74)
-SPECIAL_TROE = (0.8145, 155.8, 1675.0, 4531.0)
+# ---- a network of the cases, and each case's cell ----
 
 
-@dataclass
-class Row:
-    section: str
-    line: str
-    reactant: str
-    product: str
-    rate: object
-    partners: tuple = ()
-    product_enthalpy: float = -1000.0
-    capped: bool = False
-    bath: dict = field(default_factory=lambda: {"He": 0.5, "H2": 0.4})
-    start: float = 0.1
-    troe: bool = False
+def thermo_of(species, dH):
+    """Constant-Cp NASA-9 ranges: every species at H/R = 0 except products at their dH; He monatomic."""
+    out = {q: helpers.constant_cp(0.0, 2.5 if q == "He" else 3.5) for q in species}
+    out.update({q: helpers.constant_cp(h) for q, h in dH.items()})
+    return out
 
 
-ROWS = {
-    "Arrhenius": Row("two", "[ CO_1 + He -> CO_2 + He ] 2.0E-12 0.7 1500.0", "CO_1", "CO_2",
-                     lambda T, _: arrhenius(2e-12, 0.7, 1500.0, T), ("He",)),
-    "capped, two molecules": Row("two", "[ CO_11 + He -> CO_12 + He ] 5.0E-09 0.0 0.0", "CO_11", "CO_12",
-                                 lambda *_: 5e-9, ("He",), 1000.0, True),
-    "capped, three molecules": Row("two", "[ CO_13 + He + H2 -> CO_14 + He + H2 ] 1.0E-23 0.0 0.0", "CO_13",
-                                   "CO_14", lambda *_: 1e-23, ("He", "H2"), capped=True),
-    "capped, one molecule and M": Row("low", "[ CO_15 + M -> CO_16 + M ] 5.0E-09 0.0 0.0", "CO_15", "CO_16",
-                                      lambda *_: 5e-9, (), 1000.0, True),
-    "first order, no M": Row("two", "[ CO_17 -> CO_18 ] 1.0E+10 0.0 0.0", "CO_17", "CO_18", lambda *_: 1e10),
-    "Lindemann falloff": Row("three", "[ CO_7 + M -> CO_8 + M ] 3.0E-14 -1.0 0.0 3.16 0.5 0.0", "CO_7", "CO_8",
-                             lambda T, nd: falloff(arrhenius(*FALLOFF_K0, T), arrhenius(*FALLOFF_KINF, T), nd, T,
-                                                   None)),
-    "Troe falloff": Row("three", "[ CO_9 + M -> CO_10 + M ] 3.0E-14 -1.0 0.0 3.16 0.5 0.0\nTroe [0.6, 100.0, 2000.0, "
-                        "5000.0]", "CO_9", "CO_10",
-                        lambda T, nd: falloff(arrhenius(*FALLOFF_K0, T), arrhenius(*FALLOFF_KINF, T), nd, T, TROE),
-                        troe=True),
-    "low-pressure only": Row("low", "[ CO_3 + M -> CO_4 + M ] 3.0E-13 -0.5 800.0", "CO_3", "CO_4",
-                             lambda T, _: arrhenius(*LOW, T)),
-    "low-pressure only, efficiency": Row("low", "[ CO_5 + M -> CO_6 + M ] 3.0E-13 -0.5 800.0\neffi {'H2': 2.5, "
-                                         "'He': 0.4}", "CO_5", "CO_6", lambda T, _: 2.5 * arrhenius(*LOW, T),
-                                         bath={"H2": 1.0 - TRACE}, start=TRACE),
-    "two k0 terms, Troe of Visscher & Moses": Row(
-        "special", "[ OH + CH3 + M -> CH3OH + M ]", "OH", "CH3OH",
-        lambda T, nd: falloff(sum(arrhenius(*q, T) for q in SPECIAL_K0), arrhenius(*SPECIAL_KINF, T), nd, T,
-                              SPECIAL_TROE, vm11=True),
-        ("CH3",), bath={"CH3": 0.5, "He": 0.5 - TRACE}, start=TRACE, troe=True),
-}  # fmt: skip
-HEADINGS = {"two": "# Two-body Reactions", "three": "# 3-body and Disscoiation Reactions",
-            "low": "# 3-body reactions without high-pressure rates", "special": "# special cases"}  # fmt: skip
-
-
-def network(rows, folder):
-    sections = [
-        "\n".join([h, *(f"{2 * i + 1} {row.line}" for i, row in enumerate(rows) if row.section == s)])
-        for s, h in HEADINGS.items()
-        if any(row.section == s for row in rows)
-    ]
-    text = "# one reaction per rate law\n\n" + "\n\n".join(sections) + "\n\n# reverse stops\n"
-    thermo = {"He": helpers.constant_cp(0.0, 2.5), "H2": helpers.constant_cp(0.0)}
-    for row in rows:
-        thermo[row.reactant] = helpers.constant_cp(0.0)
-        thermo[row.product] = helpers.constant_cp(row.product_enthalpy)
-        thermo.update({q: helpers.constant_cp(0.0) for q in row.partners})
-    if "CH3OH" in thermo:
-        thermo["CH3OH"] = helpers.constant_cp(BOUND)
+def build(cases, folder, lines_of):
+    by_section = {}
+    for i, case in enumerate(cases):
+        by_section.setdefault(case["section"], []).append(f"{2 * i + 1} " + "\n".join(lines_of(case)))
+    body = "\n\n".join(SECTION_HEADINGS[s] + "\n" + "\n".join(rows) for s, rows in by_section.items())
+    text = "# one isomerisation per rate law\n\n" + body + "\n\n# reverse stops\n"
+    species = {"He", "H2"} | {q for c in cases for q in (c["from"], c["to"], *c.get("with", ()), *c.get("bath", {}))}
+    dH = {c["to"]: c.get("dH", -1000.0) for c in cases}
+    thermo = thermo_of(sorted(species), dH)
     return helpers.write_network(folder, text, thermo), thermo
 
 
-def expected(row, T, P, thermo):
-    nd = float(helpers.number_density(T, P))
-    k = row.rate(T, nd)
-    kf = k * (nd if row.section != "two" else 1.0)
-    g = {q: helpers.gibbs_rt(a, T) for q, a in thermo.items()}
-    if row.section == "special":
-        dg = g["CH3OH"] - g["OH"] - g["CH3"]
-        assert np.exp(dg) * helpers.STANDARD_PRESSURE / P * np.e / row.bath["CH3"] < 1e-12
-        K, kr = np.inf, 0.0
+def start_state(case, species):
+    """The cell's mole fractions: the reactant at 0.1 in He 0.5 and H2 0.4, or traced at TRACE in its bath."""
+    x = np.zeros(len(species))
+    if case.get("traced"):
+        bath = case["bath"]
+        for q, f in bath.items():
+            x[species.index(q)] = f * (1.0 - TRACE) / sum(bath.values())
+        x[species.index(case["from"])] = TRACE
     else:
-        K = float(np.exp(g[row.reactant] - g[row.product]))
-        kr = kf / K
-    molecules, m = 1 + len(row.partners), int(row.section != "two")
-    c = min(1.0, K_COLL / (max(k, k / K) * nd ** (molecules + m - 2))) if molecules + m >= 2 else 1.0
-    assert (c < 1.0) == row.capped
-    density = float(np.prod([nd * row.bath[q] for q in row.partners]))
-    return c * (kf + kr) * density, row.start / (1.0 + K)
+        x[species.index("He")], x[species.index("H2")], x[species.index(case["from"])] = 0.5, 0.4, 0.1
+    return x
+
+
+def folded(case, T, P, thermo):
+    """The case's forward and reverse coefficients as the call uses them (a third body's nd folded in, the
+    collision cap applied; kr = kf exp(dG/RT) (k_B T / P0)^(products - reactants), molecules without M) and the
+    isomer pair's equilibrium constant K (infinite for the special association, whose reverse is negligible)."""
+    nd = float(helpers.number_density(T, P))
+    k = coefficient(case["k"], T, nd)
+    third = case["section"] != "two"
+    g = {q: helpers.gibbs_rt(a, T) for q, a in thermo.items()}
+    association = case["section"] == "special"
+    partners = case["with"] if association else []  # spectators appear on both sides otherwise
+    dg = g[case["to"]] - g[case["from"]] - sum(g[q] for q in partners)
+    lkt = math.log(helpers.BOLTZMANN * T / helpers.STANDARD_PRESSURE)
+    ratio = math.exp(dg - len(partners) * lkt)  # kr / kf
+    if association:  # the reverse against the forward at the end of a call, traced in its bath
+        assert ratio / nd / (1.0 - TRACE) * 2.0 * math.e < 1e-12
+    molecules = 1 + len(case["with"]) + int(third)
+    cap = 1.0
+    if molecules >= 2:
+        cap = min(1.0, K_COLL / (max(k, k * ratio) * nd ** (molecules - 2)))
+    assert (cap < 1.0) == case.get("capped", False), case["name"]
+    kf = cap * k * (nd if third else 1.0)
+    return kf, kf * ratio, (math.inf if association else 1.0 / ratio)
+
+
+def spectators(case, x, species, T, P):
+    nd = float(helpers.number_density(T, P))
+    return math.prod(nd * x[species.index(q)] for q in case["with"])
+
+
+def all_cases(troe: bool):
+    return [c for c in CASES["laws"] if troe or not c["k"].get("troe")]
+
+
+@pytest.mark.parametrize("troe", [True, False], ids=["troe", "plain"])
+def test_oracle_coefficients_are_the_closed_forms(tmp_path, troe):
+    import oracle
+
+    from maxichem_amd import _tables
+
+    cases = all_cases(troe)
+    net, thermo = build(cases, tmp_path, lambda c: c["lines"])
+    orc = oracle.Oracle(_tables.build(net, photo=False))
+    rows = {r["eq"]: i for i, r in enumerate(net.reac)}
+    for case in cases:
+        q = next(i for eq, i in rows.items() if eq.split("->")[0].split()[0] == case["from"])
+        for T, P in CONDITIONS:
+            kf, kr = orc.rates(T, P, np.zeros(0))
+            want_f, want_r, _ = folded(case, T, P, thermo)
+            assert kf[q] == pytest.approx(want_f, rel=COEFFICIENT_TOLERANCE), (case["name"], T, P)
+            assert kr[q] == pytest.approx(want_r, rel=COEFFICIENT_TOLERANCE), (case["name"], T, P)
 
 
 @pytest.mark.parametrize("kind", BACKENDS)
-@pytest.mark.parametrize("form", ["troe", "plain"])
-def test_forward_rate_coefficients(tmp_path, kind, form):
-    rows = {n: r for n, r in ROWS.items() if form == "troe" or not r.troe}
-    net, thermo = network(list(rows.values()), tmp_path)
-    cases = [(n, r, T, P) for n, r in rows.items() for T, P in CONDITIONS]
-    x0 = np.zeros((len(cases), len(net.species)))
-    lam, x_eq = np.zeros(len(cases)), np.zeros(len(cases))
-    for c, (_, row, T, P) in enumerate(cases):
-        x0[c, net.species.index(row.reactant)] = row.start
-        for q, f in row.bath.items():
-            x0[c, net.species.index(q)] = f
-        lam[c], x_eq[c] = expected(row, T, P, thermo)
+@pytest.mark.parametrize("troe", [True, False], ids=["troe", "plain"])
+def test_each_law_relaxes_its_isomer_by_e_in_one_time_constant(tmp_path, kind, troe):
+    cases = all_cases(troe)
+    net, thermo = build(cases, tmp_path, lambda c: c["lines"])
+    cells = [(case, T, P) for case in cases for T, P in CONDITIONS]
+    x0 = np.array([start_state(case, net.species) for case, _, _ in cells])
+    lam, x_eq = np.empty(len(cells)), np.empty(len(cells))
+    for b, (case, T, P) in enumerate(cells):
+        kf, kr, K = folded(case, T, P, thermo)
+        lam[b] = (kf + (kr if math.isfinite(K) else 0.0)) * spectators(case, x0[b], net.species, T, P)
+        x_eq[b] = x0[b, net.species.index(case["from"])] / (1.0 + K)
     dt = 1.0 / lam
     run = helpers.backend(kind, net)
-    out = run(x0, [c[2] for c in cases], [c[3] for c in cases], dt, 1e-3 * dt, rtol=RTOL)
+    out = run(x0, [T for _, T, _ in cells], [P for _, _, P in cells], dt, 1e-3 * dt, rtol=RTOL)
     run.close()
     assert out["done"].all()
-    seen = np.array([out["x"][c, net.species.index(r.reactant)] for c, (_, r, _, _) in enumerate(cases)])
-    starts = np.array([r.start for _, r, _, _ in cases])
-    errors = np.abs(np.log((starts - x_eq) / (seen - x_eq)) / (dt * lam) - 1.0)
-    assert errors.max() < RATE_TOLERANCE, dict(zip([c[0] for c in cases], errors))
-
-
-FAR_BELOW = -2.0e5
-LIMITS = {
-    "k0 = 0": ("[ CO_1 + M -> CO_2 + M ] 0.0 0.0 0.0 3.16 0.5 0.0", "CO_1", "CO_2", None),
-    "k_inf underflows": ("[ CO_3 + M -> CO_4 + M ] 3.0E-14 -1.0 0.0 1.0E-300 0.0 2.0E+05", "CO_3", "CO_4", None),
-    "k0 = k_inf = 0": ("[ CO_5 + M -> CO_6 + M ] 0.0 0.0 0.0 0.0 0.0 0.0", "CO_5", "CO_6", None),
-    "k_inf overflows": ("[ CO_7 + M -> CO_8 + M ] 3.0E-14 -1.0 0.0 1.0E+300 20.0 0.0", "CO_7", "CO_8",
-                        lambda T, _: arrhenius(*FALLOFF_K0, T)),
-}  # fmt: skip
+    for b, (case, T, P) in enumerate(cells):
+        i = net.species.index(case["from"])
+        shrink = (out["x"][b, i] - x_eq[b]) / (x0[b, i] - x_eq[b])
+        assert abs(math.log(shrink) + 1.0) < RATE_TOLERANCE, (case["name"], T, P, shrink)
 
 
 @pytest.mark.parametrize("kind", BACKENDS)
-@pytest.mark.parametrize("form", ["troe", "plain"])
-def test_falloff_at_the_limits_of_pr(tmp_path, kind, form):
-    troe_line = "\nTroe [0.6, 100.0, 2000.0, 5000.0]" if form == "troe" else ""
-    rows = [Row("three", line + troe_line, a, b, rate or (lambda *_: 0.0), product_enthalpy=FAR_BELOW,
-                bath={"He": 0.9}) for line, a, b, rate in LIMITS.values()]  # fmt: skip
-    net, _ = network(rows, tmp_path)
-    cases = [(row, T, P) for row in rows for T, P in CONDITIONS]
-    x0 = np.zeros((len(cases), len(net.species)))
-    lam = np.zeros(len(cases))
-    for c, (row, T, P) in enumerate(cases):
-        x0[c, net.species.index(row.reactant)] = row.start
-        x0[c, net.species.index("He")] = 0.9
-        nd = float(helpers.number_density(T, P))
-        lam[c] = row.rate(T, nd) * (troe_limit(T, TROE) if form == "troe" else 1.0) * nd
+@pytest.mark.parametrize("troe", [True, False], ids=["troe", "plain"])
+def test_falloff_at_the_ends_of_pr(tmp_path, kind, troe):
+    lim = CASES["pr_limits"]
+    rows = [dict(r, section="three", k={"low": []}, dH=lim["dH"]) for r in lim["rows"]]
+    troe_line = [f"Troe {lim['troe']}"] if troe else []
+    net, _ = build(rows, tmp_path, lambda c: [c["line"], *troe_line])
+    cells = [(row, T, P) for row in rows for T, P in CONDITIONS]
+    x0 = np.zeros((len(cells), len(net.species)))
+    lam = np.zeros(len(cells))
+    for b, (row, T, P) in enumerate(cells):
+        x0[b, net.species.index(row["from"])], x0[b, net.species.index("He")] = 0.1, 0.9
+        if row["k0"] is not None:
+            nd = float(helpers.number_density(T, P))
+            f = broadening_at_low_pressure(lim["troe"], T) if troe else 1.0
+            lam[b] = modified_arrhenius(row["k0"], T) * f * nd
     dt = np.where(lam > 0.0, 1.0 / np.where(lam > 0.0, lam, 1.0), 1.0)
     run = helpers.backend(kind, net)
-    out = run(x0, [c[1] for c in cases], [c[2] for c in cases], dt, 1e-3 * dt, rtol=RTOL)
+    out = run(x0, [T for _, T, _ in cells], [P for _, _, P in cells], dt, 1e-3 * dt, rtol=RTOL)
     run.close()
     assert out["done"].all()
-    x0n = x0 / x0.sum(1, keepdims=True)
-    for c, (row, _, _) in enumerate(cases):
-        i = net.species.index(row.reactant)
-        # x = x0 exp(-lambda t), at constant P with the isomer's loss made up by its product
-        assert out["x"][c, i] == pytest.approx(x0n[c, i] * np.exp(-lam[c] * dt[c]), rel=RATE_TOLERANCE)
+    for b, (row, _, _) in enumerate(cells):
+        i = net.species.index(row["from"])
+        # at constant P the isomer's loss is made up by its product: x = x0 exp(-lambda t)
+        assert out["x"][b, i] == pytest.approx(0.1 * math.exp(-lam[b] * dt[b]), rel=RATE_TOLERANCE), row["name"]
 
 
-# ---- the collision cap against closed forms (the reference's tests/test_collision_limit.py) ----
+# ---- the collision cap on the reference's collision network, against closed forms ----
 
 K_FIRST, J_COLL, T_COLL = 1e9, 1e9, 3000.0
 P_LOW, P_HIGH = 1e3, 1e7
 
 
-def _gibbs_cp(net, name):
-    cp, h, b2 = net.lo[net.species.index(name)][[2, 7, 8]]
-    return cp + h / T_COLL - (cp * np.log(T_COLL) + b2)
+def capped_pairs(net, P):
+    """(kf, kr) of each capped reaction of tests/golden/data/synthetic_collision_network.txt at T_COLL and P: the
+    larger direction of each, in bimolecular form, held at K_COLL and the other following by K."""
+    nd = P / (helpers.BOLTZMANN * T_COLL)
+    lkt = math.log(helpers.BOLTZMANN * T_COLL / helpers.STANDARD_PRESSURE)
 
+    def g(q):  # G/RT of a constant-Cp species of the network at T_COLL
+        cp, h, b2 = net.lo[net.species.index(q)][[2, 7, 8]]
+        return cp + h / T_COLL - (cp * math.log(T_COLL) + b2)
 
-def _closed_forms(net, P):
-    t = T_COLL
-    nd, lkt = P / (helpers.BOLTZMANN * t), np.log(helpers.BOLTZMANN * t / helpers.STANDARD_PRESSURE)
-    g = {q: _gibbs_cp(net, q) for q in ("H", "H2", "O", "O2")}
-    eh = np.exp(2 * g["H"] - g["H2"] + lkt)  # kr / kf of H2 + M -> H + H + M
-    eo = np.exp(g["O2"] - 2 * g["O"] - lkt)  # kr / kf of O + O + M -> O2 + M
-    if P == P_LOW:
-        h2, o2 = (K_COLL * nd, K_COLL * nd * eh), (K_COLL * nd / eo, K_COLL * nd)
+    dissociation = math.exp(2 * g("H") - g("H2") + lkt)  # kr / kf of H2 + M -> H + H + M
+    association = math.exp(g("O2") - 2 * g("O") - lkt)  # kr / kf of O + O + M -> O2 + M
+    capped = K_COLL * nd
+    if P == P_LOW:  # the forward direction is the larger one
+        h2, o2 = (capped, capped * dissociation), (capped / association, capped)
     else:
-        h2, o2 = (K_COLL / eh, K_COLL), (K_COLL, K_COLL * eo)
-    return {"H2 + M -> H + H + M": h2, "O + O + M -> O2 + M": o2,
-            "N_1 + M -> N_2 + M": (K_COLL * nd, K_COLL * nd), "N_3 -> N_4": (K_FIRST, K_FIRST),
-            "H2O -> H + OH": (J_COLL, 0.0)}  # fmt: skip
+        h2, o2 = (K_COLL / dissociation, K_COLL), (K_COLL, K_COLL * association)
+    return {"H2 + M -> H + H + M": h2, "O + O + M -> O2 + M": o2, "N_1 + M -> N_2 + M": (capped, capped),
+            "N_3 -> N_4": (K_FIRST, K_FIRST), "H2O -> H + OH": (J_COLL, 0.0)}  # fmt: skip
 
 
 @pytest.mark.parametrize("P", [P_LOW, P_HIGH])
@@ -237,12 +242,10 @@ def test_capped_coefficients_match_closed_forms(read_network, P):
     net = read_network("synthetic_collision")
     orc = oracle.Oracle(_tables.build(net, photo=True))
     kf, kr = orc.rates(T_COLL, P, np.full(len(net.photo), J_COLL))
-    names = [r["eq"] for r in net.reac] + [r["eq"] for r in net.photo]
-    forms = _closed_forms(net, P)
-    seen = 0
-    for q, eq in enumerate(names):
-        if eq in forms:
-            seen += 1
-            for got, want in zip((kf[q], kr[q]), forms[eq]):
-                assert got == pytest.approx(want, rel=1e-13, abs=0.0), (eq, got, want)
-    assert seen == len(forms)
+    equations = [r["eq"] for r in net.reac] + [r["eq"] for r in net.photo]
+    pairs = capped_pairs(net, P)
+    assert set(pairs) <= set(equations)
+    for eq, (want_f, want_r) in pairs.items():
+        q = equations.index(eq)
+        assert kf[q] == pytest.approx(want_f, rel=1e-13, abs=0.0), eq
+        assert kr[q] == pytest.approx(want_r, rel=1e-13, abs=0.0), eq

@@ -169,14 +169,16 @@ __device__ __forceinline__ bool msolve(double (&yl)[NV], double (&yt)[NV], const
     solve_lead_forward(rl, rt, a21, ws, lane);
     solve_trailing32(rt, c, dt, lane);
     solve_lead_backward(rl, rt, dl, ws, lane);
-    double worst = 0.0;
+    // converged when every lane's correction is within REFINE_TOL weights; a NaN or infinite correction (a broken
+    // FP32 factor) fails the comparison, so the try goes to FP64
+    bool fine = true;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       yl[v] += rl[v];
       yt[v] += rt[v];
-      worst = fmax(worst, fmax(lane < K0 ? fabs(rl[v]) / wl : 0.0, lane < NT ? fabs(rt[v]) / wt : 0.0));
+      fine = fine && (lane >= K0 || fabs(rl[v]) / wl <= REFINE_TOL) && (lane >= NT || fabs(rt[v]) / wt <= REFINE_TOL);
     }
-    if (wave_max(worst) <= REFINE_TOL) return true;
+    if (wave_all(fine)) return true;
   }
   return false;
 }

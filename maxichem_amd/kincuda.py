@@ -167,9 +167,11 @@ class CudaSolver(Solver):
             xo, ho = torch.empty_like(x), h.clone()
             done, redo, lst, status = (torch.empty(B, dtype=torch.int32, device=self.dev) for _ in range(4))
             nok = torch.empty(B, dtype=torch.float64, device=self.dev)
-            order, xprop = self._plan(x, T, P, dt, ho, jx, B)
+            order, xprop, planned = self._plan(x, T, P, dt, ho, jx, B)
             self.tries = torch.zeros(B, dtype=torch.float64, device=self.dev)
             self.count.zero_()
+            if planned is not None:  # counts the plan took from the inputs before changing them
+                self.count += planned
             stream = torch.cuda.current_stream(self.dev).cuda_stream
             self._launch_all(x, T, P, dt, h, xo, ho, done, nok, status, redo, lst, jx, B, rtol, atol, max_steps,
                              stream, order, xprop)  # fmt: skip
@@ -182,12 +184,14 @@ class CudaSolver(Solver):
         """The queue's order and proposals of a call; h_start (the first step sizes) may be changed in place.
 
         The plain form queues the previous call's costliest cells first when it had as many cells (a call usually
-        follows one over the same cells) and proposes nothing. Results do not depend on the order."""
+        follows one over the same cells) and proposes nothing. Results do not depend on the order. The third value
+        is None, or counts [COUNT_SLOTS] (int32, on the device) the plan took from the inputs it changed, which the
+        call adds to its own."""
         prev = self.tries
         order = None
         if prev is not None and prev.shape[0] == B:
             order = torch.argsort(prev, descending=True).to(torch.int32)
-        return order, None
+        return order, None, None
 
     def _launch_all(self, x, T, P, dt, h, xo, ho, done, nok, status, redo, lst, jx, B, rtol, atol, max_steps,
                     stream, order, xprop=None, fp64=None) -> None:  # fmt: skip

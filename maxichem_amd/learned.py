@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _hip, _kernels
-from ._constants import H_MIN
+from ._constants import COUNT_SLOTS, H_MIN
 from .kincuda import CudaSolver
 
 TILE = 64
@@ -416,6 +416,10 @@ class LearnedSolver(CudaSolver):
         self.predicted_cost: torch.Tensor | None = None
 
     def _plan(self, x, T, P, dt, h_start, jx, B):
+        # a non-finite h the warm start replaces is counted as the call kernel counts the ones it sees
+        planned = torch.zeros_like(self.count)
+        if self.warm != "none":
+            planned[COUNT_SLOTS.index("nonfinite_h")] = (~torch.isfinite(h_start)).sum()
         prev = self.tries
         measured = prev is not None and prev.shape[0] == B
         if self.warm == "cold" and not self.propose and measured:
@@ -428,7 +432,7 @@ class LearnedSolver(CudaSolver):
                                                   H_MIN, proposals=False)  # fmt: skip
                 h_start[idx] = h0
             self.predicted_cost = None
-            return torch.argsort(prev, descending=True).to(torch.int32), None
+            return torch.argsort(prev, descending=True).to(torch.int32), None, planned
         h0, cost, xp = self.surrogate.predict(x, T, P, dt, jx, H_MIN, proposals=self.propose)
         if self.warm == "all":
             h_start.copy_(h0)
@@ -440,4 +444,4 @@ class LearnedSolver(CudaSolver):
         prev = self.tries
         key = prev if prev is not None and prev.shape[0] == B else cost
         order = torch.argsort(key, descending=True).to(torch.int32)
-        return order, (xp if self.propose else None)
+        return order, (xp if self.propose else None), planned

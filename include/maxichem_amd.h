@@ -109,6 +109,22 @@ int mxc_solver_call(mxc_solver* solver, void* stream, int cells, const double* x
                     const double* dt, const double* h, const double* J, double rtol, double atol, int max_steps,
                     double* x_out, double* h_out, int* done, double* steps, int* flag);
 
+/* The most cells one mxc_solver_call_static call takes. */
+#define MXC_STATIC_CELLS 262144
+
+/* mxc_solver_call without any device allocation: the call's work buffers (each cell's status, guard flag, recovery
+ * list entry and tries: 20 bytes per cell) are the library's own device array of MXC_STATIC_CELLS cells, one copy
+ * per device, whose address mxc_solver_open looks up; mxc_solver_call allocates them per solver instead (hipMalloc
+ * when the batch grows). Same arguments, results, counts, refusals and return codes as mxc_solver_call, bit for bit;
+ * more than MXC_STATIC_CELLS cells are refused (return 1, mxc_last_error says so) before any device work. Every
+ * call of this entry point on one device shares that array, so such calls must not overlap on a device (one stream,
+ * or synchronized between streams), whichever solver they use. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory in their call path (tight serving loops, allocation-free harnesses). */
+int mxc_solver_call_static(mxc_solver* solver, void* stream, int cells, const double* x, const double* T,
+                           const double* P, const double* dt, const double* h, const double* J, double rtol,
+                           double atol, int max_steps, double* x_out, double* h_out, int* done, double* steps,
+                           int* flag);
+
 /* Host copies of the last call's counts (int [count_slots]) and their sum since mxc_solver_open (long long
  * [count_slots]); either may be NULL. Replaces Solver::count and total (cpp/maxichem.hpp:72-73). */
 int mxc_solver_counts(const mxc_solver* solver, int* count, long long* total);

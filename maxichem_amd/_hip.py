@@ -21,7 +21,7 @@ HERE = Path(__file__).resolve().parent
 LIB = HERE / "libmaxichem_amd.so"
 SOURCES = (HERE / "csrc" / "maxichem_hip.cpp", HERE / "csrc" / "solver.cpp")
 INCLUDE = HERE.parent / "include"
-ABI_VERSION = 2
+ABI_VERSION = 3
 CACHE = Path(os.environ.get("MAXICHEM_AMD_CACHE", HERE / "_cache"))
 
 ATTR_REGISTERS, ATTR_PRIVATE_BYTES, ATTR_STATIC_LDS, ATTR_MAX_THREADS = 0, 1, 2, 3
@@ -74,6 +74,7 @@ def lib() -> ctypes.CDLL:
         h.mxc_solver_name.argtypes = [vp, i, i]
         h.mxc_solver_name.restype = ctypes.c_char_p
         h.mxc_solver_call.argtypes = [vp, vp, i, vp, vp, vp, vp, vp, vp, d, d, i, vp, vp, vp, vp, vp]
+        h.mxc_solver_call_static.argtypes = [vp, vp, i, vp, vp, vp, vp, vp, vp, d, d, i, vp, vp, vp, vp, vp]
         h.mxc_solver_counts.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_longlong)]
         h.mxc_solver_atoms.argtypes = [vp, ctypes.POINTER(d)]
         h.mxc_kernel_arguments.argtypes = [ctypes.c_char_p, ctypes.c_char_p, sz]

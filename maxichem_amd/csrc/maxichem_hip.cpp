@@ -46,7 +46,7 @@ int check_rtc(hiprtcResult e, const char* what) {
 
 extern "C" {
 
-int mxc_version(void) { return 2; }
+int mxc_version(void) { return 3; }
 
 int mxc_compiler_version(int* major, int* minor) {
   last_error.clear();

@@ -85,6 +85,12 @@ size_t dtype_size(const std::string& d) {
   return 0;
 }
 
+// The work buffers of mxc_solver_call_static, per cell: tries (double), then status, redo and list (int), for at
+// most MXC_STATIC_CELLS cells. One copy per device, held until the process ends; in host code the name stands for
+// no device address: its address on a device comes from hipGetSymbolAddress with that device current.
+constexpr size_t kStaticBytes = static_cast<size_t>(MXC_STATIC_CELLS) * (sizeof(double) + 3 * sizeof(int));
+__device__ __align__(256) unsigned char scratch[kStaticBytes];
+
 }  // namespace
 
 struct mxc_solver {
@@ -107,6 +113,7 @@ struct mxc_solver {
   int capacity = 0;  // cells the work buffers hold
   int *status = nullptr, *redo = nullptr, *list = nullptr;
   double* tries = nullptr;
+  unsigned char* static_work = nullptr;  // this device's copy of scratch (mxc_solver_call_static)
 };
 
 namespace {
@@ -276,14 +283,14 @@ int grow(mxc_solver* s, int cells) {
 }
 
 // The refusal message of the call just made (the Python solver's, maxichem/_solver.py:183-224), or empty.
-int refusal(mxc_solver* s, hipStream_t stream, int B, const double* x, const double* T, const double* P,
-            const double* dt, const double* J, std::string& message) {
+int refusal(mxc_solver* s, hipStream_t stream, int B, const int* status_d, const double* x, const double* T,
+            const double* P, const double* dt, const double* J, std::string& message) {
   const int bad = s->last[s->bad_slot], cold = s->last[s->cold_slot], hot = s->last[s->hot_slot];
   message.clear();
   if (!bad && !cold && !hot) return 0;
   std::vector<int> status(B);
   std::vector<double> Th(B);
-  if (check(hipMemcpyAsync(status.data(), s->status, sizeof(int) * B, hipMemcpyDeviceToHost, stream), "copy") ||
+  if (check(hipMemcpyAsync(status.data(), status_d, sizeof(int) * B, hipMemcpyDeviceToHost, stream), "copy") ||
       check(hipMemcpyAsync(Th.data(), T, sizeof(double) * B, hipMemcpyDeviceToHost, stream), "copy") ||
       check(hipStreamSynchronize(stream), "hipStreamSynchronize"))
     return 1;
@@ -420,6 +427,12 @@ int mxc_solver_open(const char* folder, int device, mxc_solver** solver) {
       return 1;
     }
   }
+  void* work = nullptr;  // this device's work buffers for mxc_solver_call_static (the device is current)
+  if (check(hipGetSymbolAddress(&work, HIP_SYMBOL(scratch)), "hipGetSymbolAddress")) {
+    release(s);
+    return 1;
+  }
+  s->static_work = static_cast<unsigned char*>(work);
   s->last.assign(slots, 0);
   s->total.assign(slots, 0);
   *solver = s;
@@ -462,30 +475,36 @@ const char* mxc_solver_name(const mxc_solver* s, int list, int index) {
   return (*v)[index].c_str();
 }
 
-int mxc_solver_call(mxc_solver* s, void* stream_, int cells, const double* x, const double* T, const double* P,
-                    const double* dt, const double* h, const double* J, double rtol, double atol, int max_steps,
-                    double* x_out, double* h_out, int* done, double* steps, int* flag) {
-  mxc_detail::clear_error();
-  if (!s) return fail("mxc_solver_call: null solver");
-  if (cells < 0) return fail("mxc_solver_call: cells: at least 0 expected, got " + std::to_string(cells));
+namespace {
+
+// The argument checks of a call, before any device work.
+int check_call(const mxc_solver* s, int cells, const double* x, const double* T, const double* P, const double* dt,
+               const double* h, const double* J, double rtol, double atol, int max_steps, double* x_out,
+               double* h_out, int* done, double* steps, const char* who) {
+  if (!s) return fail(std::string(who) + ": null solver");
+  if (cells < 0) return fail(std::string(who) + ": cells: at least 0 expected, got " + std::to_string(cells));
   if (cells > 0 && (!x || !T || !P || !dt || !h || !x_out || !h_out || !done || !steps))
-    return fail("mxc_solver_call: null input or output pointer");
+    return fail(std::string(who) + ": null input or output pointer");
   if (s->nj && cells > 0 && !J)
     return fail("J [B, " + std::to_string(s->nj) + "] is required with photolysis: pass zeros for dark cells");
-  if (!s->nj && J)
-    return fail("J: none expected, since this solver has no photolysis");
+  if (!s->nj && J) return fail("J: none expected, since this solver has no photolysis");
   if (!(std::isfinite(rtol) && rtol > 0.0 && std::isfinite(atol) && atol > 0.0))
     return fail("rtol and atol finite and positive expected, got " + g(rtol) + ", " + g(atol));
   const int most = std::numeric_limits<int>::max() / s->rec_steps;
   if (max_steps < 1 || max_steps > most)
     return fail("max_steps from 1 to " + std::to_string(most) + " expected, got " + std::to_string(max_steps));
-  hipStream_t stream = static_cast<hipStream_t>(stream_);
-  if (check(hipSetDevice(s->device), "hipSetDevice") || grow(s, std::max(cells, 1))) return 1;
-  int* redo = flag ? flag : s->redo;
+  return 0;
+}
+
+// A call with the given work buffers [cells] on the device: status, redo (the flags), list and tries.
+int run_call(mxc_solver* s, hipStream_t stream, int cells, const double* x, const double* T, const double* P,
+             const double* dt, const double* h, const double* J, double rtol, double atol, int max_steps,
+             double* x_out, double* h_out, int* done, double* steps, int* status, int* redo, int* list,
+             double* tries) {
   const int slots = static_cast<int>(s->counts.size());
   const size_t B = static_cast<size_t>(cells);
   if ((cells && check(hipMemcpyAsync(h_out, h, B * sizeof(double), hipMemcpyDeviceToDevice, stream), "copy h")) ||
-      (cells && check(hipMemsetAsync(s->tries, 0, B * sizeof(double), stream), "clear tries")) ||
+      (cells && check(hipMemsetAsync(tries, 0, B * sizeof(double), stream), "clear tries")) ||
       check(hipMemsetAsync(s->count, 0, slots * sizeof(int), stream), "clear counts"))
     return 1;
   const int* order = nullptr;  // cells in index order (results do not depend on the order)
@@ -496,7 +515,7 @@ int mxc_solver_call(mxc_solver* s, void* stream_, int cells, const double* x, co
   std::map<std::string, void*> v{
       {"net", &s->net},   {"x0", &x},       {"Ta", &T},        {"Pa", &P},          {"dta", &dt},
       {"hin", &h},        {"hio", &h_out},  {"xout", &x_out},  {"doneo", &done},    {"noko", &steps},
-      {"status", &s->status}, {"redo", &redo}, {"list", &s->list}, {"count", &s->count}, {"tries", &s->tries},
+      {"status", &status}, {"redo", &redo}, {"list", &list},  {"count", &s->count}, {"tries", &tries},
       {"order", &order},  {"Jin", &J},      {"B", &n},         {"rtol", &rtol},     {"atol", &atol},
       {"max_steps", &max_steps}, {"xprop", &xprop}, {"rscratch", &s->rscratch}, {"fp64", &fp64}};
   auto launch = [&](int k, hipFunction_t f, unsigned grid, unsigned block, unsigned lds) {
@@ -516,12 +535,47 @@ int mxc_solver_call(mxc_solver* s, void* stream_, int cells, const double* x, co
     return 1;
   for (int i = 0; i < slots; ++i) s->total[i] += s->last[i];
   std::string message;
-  if (refusal(s, stream, cells, x, T, P, dt, J, message)) return 1;
+  if (refusal(s, stream, cells, status, x, T, P, dt, J, message)) return 1;
   if (!message.empty()) {
     fail(message);
     return MXC_REFUSED;
   }
   return 0;
+}
+
+}  // namespace
+
+int mxc_solver_call(mxc_solver* s, void* stream_, int cells, const double* x, const double* T, const double* P,
+                    const double* dt, const double* h, const double* J, double rtol, double atol, int max_steps,
+                    double* x_out, double* h_out, int* done, double* steps, int* flag) {
+  mxc_detail::clear_error();
+  if (check_call(s, cells, x, T, P, dt, h, J, rtol, atol, max_steps, x_out, h_out, done, steps, "mxc_solver_call"))
+    return 1;
+  if (check(hipSetDevice(s->device), "hipSetDevice") || grow(s, std::max(cells, 1))) return 1;
+  return run_call(s, static_cast<hipStream_t>(stream_), cells, x, T, P, dt, h, J, rtol, atol, max_steps, x_out,
+                  h_out, done, steps, s->status, flag ? flag : s->redo, s->list, s->tries);
+}
+
+int mxc_solver_call_static(mxc_solver* s, void* stream_, int cells, const double* x, const double* T,
+                           const double* P, const double* dt, const double* h, const double* J, double rtol,
+                           double atol, int max_steps, double* x_out, double* h_out, int* done, double* steps,
+                           int* flag) {
+  mxc_detail::clear_error();
+  if (check_call(s, cells, x, T, P, dt, h, J, rtol, atol, max_steps, x_out, h_out, done, steps,
+                 "mxc_solver_call_static"))
+    return 1;
+  if (cells > MXC_STATIC_CELLS)
+    return fail("mxc_solver_call_static: at most " + std::to_string(MXC_STATIC_CELLS) + " cells per call, got " +
+                std::to_string(cells) + "; split the batch, or call mxc_solver_call");
+  if (check(hipSetDevice(s->device), "hipSetDevice")) return 1;
+  // this device's scratch, carved as its comment says: tries, then status, redo and list
+  const size_t n = static_cast<size_t>(MXC_STATIC_CELLS);
+  double* tries = reinterpret_cast<double*>(s->static_work);
+  int* status = reinterpret_cast<int*>(s->static_work + n * sizeof(double));
+  int* redo = status + n;
+  int* list = redo + n;
+  return run_call(s, static_cast<hipStream_t>(stream_), cells, x, T, P, dt, h, J, rtol, atol, max_steps, x_out,
+                  h_out, done, steps, status, flag ? flag : redo, list, tries);
 }
 
 int mxc_solver_counts(const mxc_solver* s, int* count, long long* total) {

@@ -30,9 +30,11 @@ from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
 FP64_PEAK = 78.6  # TFLOP/s, MI355X FP64 vector
-# HBM bytes the plain FP64 call kernel moves per cell-call on the 69-species workload, from FETCH_SIZE and WRITE_SIZE
-# counters of the timed calls scaled by a 512 MiB copy (profiles/mxc_call_ncho_65536_traffic.md)
-TRAFFIC_PER_CELL_CALL = {"NCHO": 2011}
+# HBM bytes the plain FP64 call kernel (its guard included) moves per cell-call on the 69-species workload: a constant
+# from the FETCH_SIZE and WRITE_SIZE counters of separate profiled runs of this bench (timed calls), scaled by what
+# they read for a 512 MiB copy (x2 and x1), profiles/mxc_call_ncho_65536_traffic.md; not measured in this run
+TRAFFIC_PER_CELL_CALL = {"NCHO": 2054}
+TRAFFIC_SOURCE = "profiles/mxc_call_ncho_65536_traffic.md (rocprofv3 FETCH_SIZE, WRITE_SIZE; a constant, not this run)"
 
 
 def parse() -> argparse.Namespace:
@@ -110,8 +112,15 @@ def cpu_baseline(tab, start: dict, first, cells: int, seconds: float, calls: int
     sample = (f"the FP64 oracle (C, OpenMP) on the first {done_calls} timed calls of {n} cells each (dt = 300 s, each "
               f"from its own previous result), {total:.1f} s" if done_calls > 1 else
               f"the FP64 oracle (C, OpenMP) on {n} evenly spaced cells of the first timed call, dt = 300 s, {t:.1f} s")
+    model = ""
+    try:
+        model = next(ln.split(":", 1)[1].strip() for ln in Path("/proc/cpuinfo").read_text().splitlines()
+                     if ln.startswith("model name"))  # fmt: skip
+    except (OSError, StopIteration):
+        pass
     return {
-        "value": n * done_calls / total, "unit": "cells/s", "cores": threads, "kind": "port", "sample": sample,
+        "value": n * done_calls / total, "unit": "cells/s", "cores": threads, "cpu_model": model, "kind": "port",
+        "sample": sample,
         "gpu_vs_oracle": {"max_rel_species_above_1e-20": rel, "max_share_diff": share,
                           "done_match": bool((dg == out["done"]).all()), "cells": n},
     }  # fmt: skip
@@ -220,7 +229,7 @@ def run_uneven(args, net, dev) -> int:
                    "variant": list(solver.info["variant"]), "seed": 3, "form": "plain"},
         "tries_per_cell": {"mean": float(tr.mean()), "p50": float(np.percentile(tr, 50)),
                            "p99": float(np.percentile(tr, 99)), "max": float(tr.max())},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK, "traffic": None, "kernel": "mxc_call (whole replay)",
                      "flop_per_try": work.per_try},
         "cpu_baseline": None,
@@ -427,11 +436,14 @@ def main() -> int:
         },
         "forms": {},
         "roofline": {
-            "bound": "mfma", "compute": "FP64 (vector FMA; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+            "bound": "fp64_valu",
+            "compute": "FP64 vector ALU (VALU; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s, and the "
+                       "kernel issues no MFMA)",
             "achieved": main_run["achieved"], "peak": FP64_PEAK, "unit": "TFLOP/s",
             "frac": main_run["achieved"] / FP64_PEAK,
             "traffic": (TRAFFIC_PER_CELL_CALL[args.network] * args.cells
                         if headline == "plain" and args.network in TRAFFIC_PER_CELL_CALL else None),
+            "traffic_source": TRAFFIC_SOURCE if headline == "plain" and args.network in TRAFFIC_PER_CELL_CALL else None,
             "kernel": "mxc_call", "kernel_ms": float(main_run["kernel_ms"].mean()),
             "flop_per_try": work.per_try, "flop_rates_per_cell": work.rates,
             "tries_per_launch": float(np.mean([t.sum() for t in main_run["tries"]])),

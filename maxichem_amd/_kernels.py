@@ -96,6 +96,14 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixe
         "TMIN": C.T_MIN, "TMAX": C.T_MAX, "XSUMTOL": C.X_SUM_TOL, "DRIFT": float(drift),
         "ETOT": float(tab.atoms.sum()), "KCOLL": C.K_COLL, "KBOLTZ": C.KB, "P0BAR": C.P0,
     }  # fmt: skip
+    # the tangent kernel's region per cell after the call's (kernels/jvp.hip): rate tangents [2 NR], then the
+    # hyper-dual n [4][NS + 1] (G / RT and its tangent there while the rates are computed), v [4][NR + 1], pieces
+    # [4][NPART]
+    jn = 2 * tab.nr
+    jv = jn + 4 * (tab.ns + 1)
+    jp = jv + 4 * (tab.nr + 1)
+    jsize = jp + 4 * tab.npart
+    ints.update({"J_KD": 0, "J_N": jn, "J_G": jn, "J_V": jv, "J_PART": jp, "J_SIZE": jsize + (jsize & 1)})
     lines = [f"// network header: {tab.ns} species, {tab.nr} reactions, tables {tab.digest()}"]
     lines += [f"constexpr int {k} = {v};" for k, v in ints.items()]
     lines += [f"constexpr double {k} = {v!r};" for k, v in doubles.items()]
@@ -104,6 +112,9 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixe
         f"constexpr unsigned SV_R_MASK = {(1 << SV_R_BITS) - 1}u;",
         f"__device__ constexpr int CH_ROUNDS[{nch}] = {{{', '.join(map(str, rounds))}}};",
         f"__device__ constexpr int CH_OFF[{nch}] = {{{', '.join(map(str, offsets))}}};",
+        # the trailing rows' filled pattern: bit j of TRAIL_FILL[t] is column K0 + j of row K0 + t
+        f"__device__ constexpr unsigned long long TRAIL_FILL[{tab.nt}] = "
+        f"{{{', '.join(f'{m}ull' for m in tab.trail_fill)}}};",
         f"#define HAS_TROE {int(tab.has_troe)}",
         f"#define MIXED {int(mixed)}",
         f"#define CALL_WAVES_PER_SIMD {int(wps)}",
@@ -112,6 +123,12 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixe
         "#define DNAN __longlong_as_double(0x7ff8000000000000ll)",
     ]
     return "\n".join(lines) + "\n"
+
+
+def jvp_lds_bytes(tab: Tables, cpw: int) -> int:
+    """Dynamic LDS of a tangent workgroup (kernels/jvp.hip): each cell's call region and its tangent region."""
+    jsize = 2 * tab.nr + 4 * (tab.ns + 1) + 4 * (tab.nr + 1) + 4 * tab.npart
+    return 8 * (tab.layout()["SIZE"] + jsize + (jsize & 1)) * cpw
 
 
 def lds_bytes(tab: Tables, cpw: int, mixed: bool = False, rates_global: bool = False) -> tuple[int, int]:
@@ -151,10 +168,10 @@ def mlp_source() -> str:
 
 
 def source(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixed: bool = False, wps: int = 1,
-           rates_global: bool = False) -> str:
+           rates_global: bool = False, extra: tuple[str, ...] = ()) -> str:
     """The whole translation unit hiprtc compiles: the header, then the kernel files in order (mixed.hip after
-    common.hip in the mixed-precision form)."""
+    common.hip in the mixed-precision form), then the extra kernel files (the tangent's jvp.hip)."""
     parts = [header(tab, cpw, drift, batch, mixed, wps, rates_global)]
-    for name in KERNEL_FILES[:1] + (("mixed.hip",) if mixed else ()) + KERNEL_FILES[1:]:
+    for name in KERNEL_FILES[:1] + (("mixed.hip",) if mixed else ()) + KERNEL_FILES[1:] + tuple(extra):
         parts.append(f"// ---- {name} ----\n" + (KERNEL_DIR / name).read_text())
     return "\n".join(parts)

@@ -253,6 +253,16 @@ def build(net: VulcanNetwork, photo: bool = False) -> Tables:
     pos[perm] = np.arange(ns)
     nt = min(ns, WAVE)
     k0 = ns - nt
+    # the unpivoted LU's filled pattern in the elimination order; the trailing rows' part as one bit mask per row
+    # (bit j: column k0 + j), which the trailing factorization reads to leave out updates by structural zeros
+    fill = np.zeros((ns, ns), bool)
+    for i, j in pattern:
+        fill[pos[i], pos[j]] = True
+    for k in range(ns):
+        rows = np.flatnonzero(fill[k + 1 :, k]) + k + 1
+        cols = np.flatnonzero(fill[k, k + 1 :]) + k + 1
+        fill[np.ix_(rows, cols)] = True
+    trail_fill = [sum(1 << int(j) for j in np.flatnonzero(fill[k0 + t, k0:])) for t in range(nt)]
     tab = Tables(
         ns=ns, nr=nr, nrt=nrt, nj=nr - nrt, nel=len(net.elements), k0=k0, nt=nt,
         perm=np.asarray(perm, np.int32), rx=rx, rc=rc, rf=rf,
@@ -261,6 +271,7 @@ def build(net: VulcanNetwork, photo: bool = False) -> Tables:
         has_troe=any("troe" in r for r in net.reac), has_eff=any("eff" in r for r in net.reac),
     )  # fmt: skip
     tab.ndv, tab.live = ndv, len(live)
+    tab.trail_fill = trail_fill
     if ndv >= (1 << T_IDX_BITS) - 1 or nr >= (1 << SV_R_BITS):
         raise ValueError(f"the network is too large for the kernel's streams: {ndv} derivative terms, {nr} reactions")
     # reaction words: species in elimination order (pad ns), dv offset

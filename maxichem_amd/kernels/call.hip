@@ -35,13 +35,20 @@ __device__ __forceinline__ bool solve_m0(double (&yl)[NV], double (&yt)[NV], con
 }
 #endif
 
+// What a try's stages leave besides K1, K3 and K4, for a tangent that reuses its factors: K2, M0^-1 w (lead / trailing
+// parts), the Sherman-Morrison factor c0 and the factors' 1 / U_ii of this lane's lead row.
+struct StageKeep {
+  double K2l, K2t, Ul, Ut, c0, dl;
+};
+
 // One try's stages from the state N at the step ha: F, w and -Jn at N, M0 = I / (gamma ha) - Jn factored, then K1 .. K4
 // (the lead / trailing parts). Returns false when a solve's FP32 refinement did not converge (the mixed form only).
-template <typename T>
+// keep, when not null, receives StageKeep's values (the arithmetic is the same either way).
+template <typename T, typename DT = T>
 __device__ __forceinline__ bool stages(T (&c)[NT], LeadRow& a21, double Nl, double Nt, double ha, double rtol,
                                        double atol, const Net* net, double nd, double* ws, const double* rk,
                                        int lane, double& K1l, double& K1t, double& K3l, double& K3t, double& K4l,
-                                       double& K4t) {
+                                       double& K4t, StageKeep* keep = nullptr, DT* dtr_keep = nullptr) {
   const double igh = 1.0 / (GAMMA * ha), ih = 1.0 / ha;
   double Fl = 0.0, Ft = 0.0, ul = 0.0, ut = 0.0, u0l = 0.0, u0t = 0.0;
   const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
@@ -84,6 +91,8 @@ __device__ __forceinline__ bool stages(T (&c)[NT], LeadRow& a21, double Nl, doub
   sk = vsum(y1l[0], y1t[0], lane) * c0;
   K4l = y1l[0] + sk * Ul;
   K4t = y1t[0] + sk * Ut;
+  if (keep) *keep = {K2l, K2t, Ul, Ut, c0, dl};
+  if (dtr_keep) *dtr_keep = dtr;
   return true;
 }
 

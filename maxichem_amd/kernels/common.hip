@@ -551,7 +551,8 @@ __device__ __forceinline__ void factor_trailing(double (&c)[NT], double& dt, dou
       // at most BROADCAST_WINDOW of row k's broadcasts ahead of their updates: hoisted further, they would hold up
       // to 126 scalar registers and spill them to vector lanes
       if (j >= k + 1 + BROADCAST_WINDOW) tie(c[j], c[j - BROADCAST_WINDOW]);
-      c[j] = fma(-l, readlane(c[j], k), c[j]);
+      // U_kj is a structural zero of the filled pattern (exactly 0.0): no broadcast, no update
+      if ((TRAIL_FILL[k] >> j) & 1ull) c[j] = fma(-l, readlane(c[j], k), c[j]);
     }
   }
   wave_sync();

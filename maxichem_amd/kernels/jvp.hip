@@ -1,0 +1,508 @@
+// The tangent (JVP) of a call under the frozen contract: the call's own tries, bit for bit (the call kernel's device
+// functions in the call kernel's order), and along them the tangent of the state through every accepted try, with
+// the accepted step sizes held fixed (a rejected try drops out), then through the guard's normalisation and share
+// correction. The inputs' tangents: x, T, P and J (the reference's tangent kernel, maxichem/kernels/jvp.cuh, and its
+// contract in docs/method.md, "Derivatives"; here only the frozen one).
+//
+// An accepted try of step h from N solves W K_i = b_i with W = I / (gamma h) - dF/dN at N. Its tangent, with h fixed:
+//   W Kdot_i = bdot_i + Jdot K_i,
+// where Jdot K_i, the Jacobian's derivative along the tangent applied to K_i, is the eps-del part of
+// F(N + eps Ndot + del K_i; rates + eps ratesdot), and bdot_1 = Fdot(N), bdot_2 = Fdot(N) + 4 Kdot_1 / h,
+// bdot_3 = Fdot(Y3)[Ydot3] + (Kdot_1 - Kdot_2) / h, bdot_4 = Fdot(Y4)[Ydot4] + (Kdot_1 - Kdot_2 - 8/3 Kdot_3) / h.
+// The solves reuse the try's factors of M0 and its Sherman-Morrison vector. The state's tangent is then
+// Ndot + 2 Kdot_1 + Kdot_3 + Kdot_4 where the new state is not clamped at 0 (and 0 where it is).
+//
+// F is evaluated in hyper-dual numbers (value, eps, del, eps-del parts) reaction by reaction: n = nd Y / sum Y, every
+// rate a product of its coefficient and up to three n, S v by the rhs's lane streams, F = (sum Y / nd) S v.
+
+// ---- dual numbers (value, first derivative along the tangent) for the rate coefficients ----
+
+struct Dual {
+  double v, d;
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, fma(a.d, b.v, a.v * b.d)}; }
+__device__ __forceinline__ Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+__device__ __forceinline__ Dual dexp(Dual a) {
+  const double e = exp(a.v);
+  return {e, e * a.d};
+}
+__device__ __forceinline__ Dual dlog10(Dual a) { return {log10(a.v), a.d / (a.v * 2.302585092994045684)}; }
+__device__ __forceinline__ Dual dexp10(Dual a) {
+  const double e = exp10(a.v);
+  return {e, e * a.d * 2.302585092994045684};
+}
+
+// The rate law of row q at T (with its tangent) and nd (with its tangent): the call's law() in dual numbers.
+__device__ __forceinline__ Dual law_dual(const double* __restrict__ q, int fl, Dual T, Dual lT, Dual iT, Dual nd) {
+  Dual k = dexp(Dual{q[0], 0.0} + q[1] * lT - q[2] * iT);
+  if (fl & FL_SECOND_K0) k = k + dexp(Dual{q[3], 0.0} + q[4] * lT - q[5] * iT);
+  if (fl & FL_FALLOFF) {
+    const Dual kinf = dexp(Dual{q[6], 0.0} + q[7] * lT - q[8] * iT);
+    const Dual pr = k * nd / kinf;
+    k = k.v > 0.0 ? k / (Dual{1.0, 0.0} + pr) : Dual{0.0, 0.0};
+#if HAS_TROE
+    if ((fl & FL_TROE) && k.v > 0.0) {
+      const Dual one{1.0, 0.0};
+      const Dual fc = dlog10((Dual{1.0 - q[9], 0.0}) * dexp(Dual{0.0, 0.0} - T / Dual{q[10], 0.0}) +
+                             Dual{q[9], 0.0} * dexp(Dual{0.0, 0.0} - T / Dual{q[11], 0.0}) +
+                             dexp(Dual{-q[12], 0.0} / T));
+      Dual y;
+      if (fl & FL_VM11) {
+        y = dlog10(pr) / (Dual{TROE_N0, 0.0} - TROE_N1 * fc);
+      } else if (isfinite(log10(pr.v))) {
+        const Dual x = dlog10(pr) - Dual{TROE_C0, 0.0} - TROE_C1 * fc;
+        y = x / (Dual{TROE_N0, 0.0} - TROE_N1 * fc - TROE_D * x);
+      } else {
+        y = Dual{-1.0 / TROE_D, 0.0};
+      }
+      k = k * dexp10(fc / (one + y * y));
+    }
+#endif
+  }
+#if HAS_EFF
+  if (fl & FL_EFF) k = q[13] * k;
+#endif
+  return k;
+}
+
+// The tangents of every reaction's coefficients (kfdot, krdot into rkd [2 NR]) from the tangents of T, nd and J (Jd).
+// The coefficients follow the call's cell_rates: G / RT from NASA-9, whose T-derivative is -H / (R T^2); the
+// collision limit's factor takes the tangent of the larger direction (half of each at a tie).
+__device__ __forceinline__ void rates_tangent(double* ts, double* rkd, double T, double Td, double nd, double ndd, const Net* net,
+                              const double* Jdc, int lane) {
+  net = opaque(net);
+  const double* rc = net->rc;
+  const int* rf = net->rf;
+  const u64* rxe = net->rxe;
+  double* g = ts + J_G;    // G / RT
+  double* gd = g + NS + 1;  // its tangent
+  const double lT = log(T), iT = 1.0 / T, lKT = log(KBOLTZ * T / P0BAR);
+  const double* cf = net->nasa + (T < NASA_SPLIT_T ? 0 : NS * 9);
+  for (int i = lane; i < NS; i += WAVE) {
+    const double* a = cf + 9 * i;
+    g[i] = gibbs_rt(a, T, lT, iT);
+    const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+    const double h = -a[0] * iT * iT + a[1] * lT * iT + a[2] + a[3] * T / 2 + a[4] * T2 / 3 + a[5] * T3 / 4 +
+                     a[6] * T4 / 5 + a[7] * iT;  // H / RT
+    gd[i] = -h * iT * Td;
+  }
+  if (lane == 0) g[NS] = gd[NS] = 0.0;
+  wave_sync();
+  const Dual Tq{T, Td}, lTq{lT, Td * iT}, iTq{iT, -Td * iT * iT}, ndq{nd, ndd};
+  const double lKTd = Td * iT;
+  for (int r = lane; r < NR; r += WAVE) {
+    const int fl = rf[r];
+    Dual f{0.0, 0.0}, b{0.0, 0.0};
+    if (fl & FL_PHOTO) {
+      f = Dual{0.0, Jdc ? Jdc[r - NRT] : 0.0};
+    } else {
+      const u64 w = rxe[r];
+      const Dual k = law_dual(rc + RCW * r, fl, Tq, lTq, iTq, ndq);
+      const int s0 = species_of(w, 0), s1 = species_of(w, 1), s2 = species_of(w, 2);
+      const int s3 = species_of(w, 3), s4 = species_of(w, 4), s5 = species_of(w, 5);
+      const int mf = (fl >> FL_NF_SHIFT) & 3, mr = (fl >> FL_NR_SHIFT) & 3;
+      const Dual dg{g[s3] + g[s4] + g[s5] - g[s0] - g[s1] - g[s2], gd[s3] + gd[s4] + gd[s5] - gd[s0] - gd[s1] - gd[s2]};
+      const bool tb = (fl & FL_THREE_BODY) != 0;
+      const Dual c3 = tb ? ndq : Dual{1.0, 0.0};
+      f = k * c3;
+      b = (k * dexp(dg + (double)(mr - mf) * Dual{lKT, lKTd})) * c3;
+      auto bim = [&](int m, Dual kk) {
+        return m >= 2 ? kk * (m == 3 ? ndq : Dual{1.0, 0.0}) : (tb ? kk / ndq : Dual{0.0, 0.0});
+      };
+      const Dual qf = bim(mf, f), qr = bim(mr, b);
+      const Dual qb = qf.v > qr.v ? qf : (qr.v > qf.v ? qr : Dual{qf.v, 0.5 * (qf.d + qr.d)});
+      if (qb.v > KCOLL) {
+        const Dual c = Dual{KCOLL, 0.0} / qb;
+        f = f * c;
+        b = b * c;
+      }
+    }
+    rkd[r] = f.d;
+    rkd[NR + r] = b.d;
+  }
+  wave_sync();
+}
+
+// ---- F in hyper-dual numbers ----
+
+// A hyper-dual number: value, eps part, del part, eps-del part.
+struct Hyper {
+  double v, e, d, ed;
+};
+__device__ __forceinline__ Hyper hmul(Hyper a, Hyper b) {
+  return {a.v * b.v, fma(a.e, b.v, a.v * b.e), fma(a.d, b.v, a.v * b.d),
+          fma(a.ed, b.v, fma(a.e, b.d, fma(a.d, b.e, a.v * b.ed)))};
+}
+
+// The eps part (Fe) of F(Y + eps Ye; rates + eps ratesdot, nd + eps ndd) and, with HD, the eps-del part (Fed) of
+// F(Y + eps Ye + del K; ...): lead / trailing parts per lane. Uses the tangent region ts (n, v and pieces, four parts
+// each) and the call's rate coefficients rk and their tangents rkd.
+template <bool HD>
+__device__ __forceinline__ void rhs_tangent(double Yl, double Yt, double Yel, double Yet, double Kl, double Kt, double& Fel,
+                            double& Fet, double& Fedl, double& Fedt, const Net* net, double nd, double ndd,
+                            double* ts, const double* rk, const double* rkd, int lane) {
+  const bool hl = lane < K0, ht = lane < NT;
+  double sums[3] = {(hl ? Yl : 0.0) + (ht ? Yt : 0.0), (hl ? Yel : 0.0) + (ht ? Yet : 0.0),
+                    HD ? (hl ? Kl : 0.0) + (ht ? Kt : 0.0) : 0.0};
+  wave_sums(sums);
+  const double nt = sums[0], nte = sums[1], ntd = sums[2];
+  // x = nd / ntot in hyper-duals: nd has an eps part, 1 / ntot all four
+  const double in = 1.0 / nt;
+  const Hyper inv{in, -nte * in * in, -ntd * in * in, 2.0 * nte * ntd * in * in * in};
+  const Hyper xf = hmul(Hyper{nd, ndd, 0.0, 0.0}, inv);
+  double* n = ts + J_N;  // [4][NS + 1]
+  constexpr int SN = NS + 1;
+  auto put = [&](int i, double y, double ye, double kd) {
+    const Hyper v = hmul(xf, Hyper{y, ye, kd, 0.0});
+    n[i] = v.v;
+    n[SN + i] = v.e;
+    n[2 * SN + i] = v.d;
+    n[3 * SN + i] = v.ed;
+  };
+  if (hl) put(lane, Yl, Yel, HD ? Kl : 0.0);
+  if (ht) put(K0 + lane, Yt, Yet, HD ? Kt : 0.0);
+  if (lane == 0) {
+    n[NS] = 1.0;
+    n[SN + NS] = n[2 * SN + NS] = n[3 * SN + NS] = 0.0;
+  }
+  wave_sync();
+  net = opaque(net);
+  const u64* rxe = net->rxe;
+  const u32* svw = net->svw;
+  const u32* rowp = net->rowp;
+  double* V = ts + J_V;  // [4][NR + 1]
+  constexpr int SV = NR + 1;
+  auto nh = [&](int i) { return Hyper{n[i], n[SN + i], HD ? n[2 * SN + i] : 0.0, HD ? n[3 * SN + i] : 0.0}; };
+  for (int r = lane; r < NR; r += WAVE) {
+    const u64 w = rxe[r];
+    const Hyper pf = hmul(hmul(nh(species_of(w, 0)), nh(species_of(w, 1))), nh(species_of(w, 2)));
+    const Hyper pr = hmul(hmul(nh(species_of(w, 3)), nh(species_of(w, 4))), nh(species_of(w, 5)));
+    const Hyper vf = hmul(Hyper{rk[r], rkd[r], 0.0, 0.0}, pf);
+    const Hyper vr = hmul(Hyper{rk[NR + r], rkd[NR + r], 0.0, 0.0}, pr);
+    V[r] = vf.v - vr.v;
+    V[SV + r] = vf.e - vr.e;
+    V[2 * SV + r] = vf.d - vr.d;
+    V[3 * SV + r] = vf.ed - vr.ed;
+  }
+  if (lane == 0) V[NR] = V[SV + NR] = V[2 * SV + NR] = V[3 * SV + NR] = 0.0;
+  wave_sync();
+  // S v by the rhs's lane streams, four parts at once
+  double* part = ts + J_PART;  // [4][NPART]
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int q = 0; q < SV_ROUNDS; q++) {
+    const u32 w = svw[q * WAVE + lane];
+    const int r = (int)(w & SV_R_MASK);
+    const double c = coefficient(w, SV_NEG, SV_DBL);
+#pragma unroll
+    for (int p = 0; p < 4; p++) acc[p] = fma(c, V[p * SV + r], acc[p]);
+    if ((w >> SV_END) & 1u) {
+      const int piece = (int)(w >> SV_PIECE);
+#pragma unroll
+      for (int p = 0; p < 4; p++) {
+        part[p * NPART + piece] = acc[p];
+        acc[p] = 0.0;
+      }
+    }
+  }
+  wave_sync();
+  // F = (ntot / nd) S v in hyper-duals
+  const double ind = 1.0 / nd;
+  const Hyper ixf = hmul(Hyper{nt, nte, ntd, 0.0}, Hyper{ind, -ndd * ind * ind, 0.0, 0.0});
+  auto row = [&](int k, double& fe, double& fed) {
+    const u32 rp = rowp[k];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int e = 0; e < (int)(rp >> 16); e++) {
+#pragma unroll
+      for (int p = 0; p < 4; p++) s[p] += part[p * NPART + (rp & 0xffffu) + e];
+    }
+    const Hyper F = hmul(ixf, Hyper{s[0], s[1], s[2], s[3]});
+    fe = F.e;
+    fed = F.ed;
+  };
+  Fel = Fet = Fedl = Fedt = 0.0;
+  if (hl) row(lane, Fel, Fedl);
+  if (ht) row(K0 + lane, Fet, Fedt);
+  wave_sync();
+}
+
+// W^-1 y (lead / trailing parts) with a try's factors: M0^-1 y, then the Sherman-Morrison term of w 1^T.
+__device__ __forceinline__ void w_solve(double& yl, double& yt, const double (&c)[NT], const LeadRow& a21,
+                                        const StageKeep& kp, double dtr, double* ws, int lane) {
+  double vl[1] = {yl}, vt[1] = {yt};
+  solve<1>(vl, vt, c, a21, kp.dl, dtr, ws, lane);
+  const double sk = vsum(vl[0], vt[0], lane) * kp.c0;
+  yl = vl[0] + sk * kp.Ul;
+  yt = vt[0] + sk * kp.Ut;
+}
+
+// The tangent of the guard's result x' (network order, into out) from the stepped state N and its tangent, and the
+// input x0 and its tangent x0d, for a cell that passed the guard: x = N / sum N, then x' = x g / sum x g with
+// g = clamp(1 + sum_e lambda_e a_e) and (A^T X A) lambda = e0 s1 / s0 - e1 (live elements as the guard takes them).
+__device__ __forceinline__ void guard_tangent(double Nl, double Nt, double Ndl, double Ndt, const double* __restrict__ x0,
+                              const double* __restrict__ x0d, double* __restrict__ out, const Net* net, int lane) {
+  net = opaque(net);
+  const int* perm = net->perm;
+  const double* atoms = net->atoms;
+  const bool hl = lane < K0, ht = lane < NT;
+  const int il = hl ? perm[lane] : 0, it = ht ? perm[K0 + lane] : 0;
+  double Al[NEL], At[NEL];
+#pragma unroll
+  for (int e = 0; e < NEL; e++) {
+    Al[e] = hl ? atoms[il * NEL + e] : 0.0;
+    At[e] = ht ? atoms[it * NEL + e] : 0.0;
+  }
+  double ss[2] = {(hl ? Nl : 0.0) + (ht ? Nt : 0.0), (hl ? Ndl : 0.0) + (ht ? Ndt : 0.0)};
+  wave_sums(ss);
+  const double s = ss[0], sd = ss[1];
+  const double xl = hl ? Nl / s : 0.0, xt = ht ? Nt / s : 0.0;
+  const double xdl = hl ? (Ndl - xl * sd) / s : 0.0, xdt = ht ? (Ndt - xt * sd) / s : 0.0;
+  const double al = hl ? x0[il] : 0.0, at = ht ? x0[it] : 0.0;
+  const double adl = hl ? x0d[il] : 0.0, adt = ht ? x0d[it] : 0.0;
+  // e0, e1 and their tangents
+  double e[4 * NEL];
+#pragma unroll
+  for (int k = 0; k < NEL; k++) {
+    e[k] = fma(Al[k], al, At[k] * at);
+    e[NEL + k] = fma(Al[k], xl, At[k] * xt);
+    e[2 * NEL + k] = fma(Al[k], adl, At[k] * adt);
+    e[3 * NEL + k] = fma(Al[k], xdl, At[k] * xdt);
+  }
+  wave_sums(e);
+  double s0 = 0.0, s1 = 0.0, s0d = 0.0, s1d = 0.0;
+#pragma unroll
+  for (int k = 0; k < NEL; k++) {
+    s0 += e[k];
+    s1 += e[NEL + k];
+    s0d += e[2 * NEL + k];
+    s1d += e[3 * NEL + k];
+  }
+  // A^T X A and its tangent
+  double m[2 * NSYM];
+#pragma unroll
+  for (int a = 0; a < NEL; a++) {
+#pragma unroll
+    for (int b = a; b < NEL; b++) {
+      m[sym_index(a, b)] = fma(Al[a] * Al[b], xl, (At[a] * At[b]) * xt);
+      m[NSYM + sym_index(a, b)] = fma(Al[a] * Al[b], xdl, (At[a] * At[b]) * xdt);
+    }
+  }
+  wave_sums(m);
+  const double r = s1 / s0, rd = (s1d - r * s0d) / s0;
+  double y[NEL], yd[NEL], d0[NEL], lam[NEL], lamd[NEL], l[NEL * NEL];
+  bool live[NEL];
+#pragma unroll
+  for (int k = 0; k < NEL; k++) {
+    y[k] = fma(e[k], r, -e[NEL + k]);
+    yd[k] = e[2 * NEL + k] * r + e[k] * rd - e[3 * NEL + k];
+    d0[k] = m[sym_index(k, k)];
+  }
+  // the guard's elimination on M (its multipliers kept), then both right-hand sides
+  double mm[NSYM];
+#pragma unroll
+  for (int k = 0; k < NSYM; k++) mm[k] = m[k];
+#pragma unroll
+  for (int k = 0; k < NEL; k++) {
+    live[k] = mm[sym_index(k, k)] > ELEMENT_PIVOT_MIN * d0[k];
+#pragma unroll
+    for (int j = k + 1; j < NEL; j++) {
+      l[k * NEL + j] = live[k] ? mm[sym_index(k, j)] / mm[sym_index(k, k)] : 0.0;
+#pragma unroll
+      for (int f = j; f < NEL; f++) mm[sym_index(j, f)] = fma(-l[k * NEL + j], mm[sym_index(k, f)], mm[sym_index(j, f)]);
+    }
+  }
+  auto solve_sym = [&](double (&b)[NEL], double (&z)[NEL]) {
+#pragma unroll
+    for (int k = 0; k < NEL; k++) {
+#pragma unroll
+      for (int j = k + 1; j < NEL; j++) b[j] = fma(-l[k * NEL + j], b[k], b[j]);
+    }
+#pragma unroll
+    for (int k = NEL - 1; k >= 0; k--) {
+      double v = b[k];
+#pragma unroll
+      for (int f = k + 1; f < NEL; f++) v = fma(-mm[sym_index(k, f)], z[f], v);
+      z[k] = live[k] ? v / mm[sym_index(k, k)] : 0.0;
+    }
+  };
+  solve_sym(y, lam);
+  // M lamd = yd - Mdot lam, over the live elements
+  double t[NEL];
+#pragma unroll
+  for (int a = 0; a < NEL; a++) {
+    double v = yd[a];
+#pragma unroll
+    for (int b = 0; b < NEL; b++) v -= m[NSYM + (a <= b ? sym_index(a, b) : sym_index(b, a))] * lam[b];
+    t[a] = v;
+  }
+  solve_sym(t, lamd);
+  double gl = 1.0, gt = 1.0, gdl = 0.0, gdt = 0.0;
+#pragma unroll
+  for (int k = 0; k < NEL; k++) {
+    if (Al[k] != 0.0) {
+      gl = fma(lam[k], Al[k], gl);
+      gdl = fma(lamd[k], Al[k], gdl);
+    }
+    if (At[k] != 0.0) {
+      gt = fma(lam[k], At[k], gt);
+      gdt = fma(lamd[k], At[k], gdt);
+    }
+  }
+  // a clamp passes its tangent at the bound, and holds it beyond
+  if (gl < 1.0 - GMAX || gl > 1.0 + GMAX) gdl = 0.0;
+  if (gt < 1.0 - GMAX || gt > 1.0 + GMAX) gdt = 0.0;
+  gl = fmin(fmax(gl, 1.0 - GMAX), 1.0 + GMAX);
+  gt = fmin(fmax(gt, 1.0 - GMAX), 1.0 + GMAX);
+  const double yl = xl * gl, yt = xt * gt, ydl = fma(xdl, gl, xl * gdl), ydt = fma(xdt, gt, xt * gdt);
+  double S[2] = {(hl ? yl : 0.0) + (ht ? yt : 0.0), (hl ? ydl : 0.0) + (ht ? ydt : 0.0)};
+  wave_sums(S);
+  if (hl) out[il] = (ydl - yl / S[0] * S[1]) / S[0];
+  if (ht) out[it] = (ydt - yt / S[0] * S[1]) / S[0];
+}
+
+// The call and its frozen tangent: the call kernel's loop (no proposals, FP64) with the tangent carried through each
+// accepted try, the guard with its tangent. Inputs' tangents: x0d [B][NS], Td, Pd [B], Jd [B][NJ] (may be null).
+// Output: xd [B][NS], the tangent of the guard's result (network order); a cell the guard fails gets NaN there (the
+// recovery recomputes it without a tangent).
+extern "C" __global__ void __launch_bounds__(WAVE* CPW)
+    mxc_jvp(const Net* __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
+            const double* __restrict__ Pa, const double* __restrict__ dta, double* __restrict__ hio,
+            double* __restrict__ xout, int* __restrict__ doneo, double* __restrict__ noko, int* __restrict__ status,
+            int* __restrict__ redo, int* __restrict__ list, int* __restrict__ count, double* __restrict__ tries,
+            const double* __restrict__ Jin, int B, double rtol, double atol, int max_steps,
+            const double* __restrict__ x0d, const double* __restrict__ Td, const double* __restrict__ Pd,
+            const double* __restrict__ Jd, double* __restrict__ xd) {
+  extern __shared__ double lds[];
+  const int lane = (int)(threadIdx.x & (WAVE - 1)), wv = (int)(threadIdx.x / WAVE);
+  double* ws = lds + wv * (L_SIZE + J_SIZE);
+  double* ts = ws + L_SIZE;
+  double* rk = ws + L_KF;
+  double* rkd = ts + J_KD;
+  const int waves = (int)gridDim.x * CPW;
+  int q = (int)blockIdx.x * CPW + wv;
+  double c[NT];
+  LeadRow a21;
+  while (q < B) {
+    const int cell = q;
+    const double Tc = Ta[cell], dt = dta[cell], nd = Pa[cell] / (KBOLTZ * Tc);
+    const double* xr = x0 + (size_t)cell * NS;
+    const double* xdr = x0d + (size_t)cell * NS;
+    const int* perm = opaque(net)->perm;
+    double Nl = lane < K0 ? xr[perm[lane]] : 0.0, Nt = lane < NT ? xr[perm[K0 + lane]] : 0.0;
+    double Ndl = lane < K0 ? xdr[perm[lane]] : 0.0, Ndt = lane < NT ? xdr[perm[K0 + lane]] : 0.0;
+    // the call kernel's screen
+    double p = (lane < K0 ? (Nl >= 0.0 ? Nl : DNAN) : 0.0) + (lane < NT ? (Nt >= 0.0 ? Nt : DNAN) : 0.0);
+    double pm = fmax(lane < K0 ? Nl : 0.0, lane < NT ? Nt : 0.0);
+    const double* Jc = NJ ? Jin + (size_t)cell * NJ : nullptr;
+    for (int b = lane; b < NJ; b += WAVE) p += Jc[b] >= 0.0 ? 0.0 * Jc[b] : DNAN;
+    const double ps = wave_sum(p), pmax = wave_max(pm);
+    const bool fine = fabs(ps - 1.0) <= XSUMTOL && pmax > 0.0 && isfinite(Tc) && isfinite(nd) && nd > 0.0 &&
+                      isfinite(dt) && dt >= 0.0;
+    const int st = __builtin_amdgcn_readfirstlane(
+        Tc < TMIN ? REFUSED_COLD : (Tc > TMAX ? REFUSED_HOT : (fine ? 0 : REFUSED_BAD_INPUT)));
+    if (lane == 0) status[cell] = st;
+    // nd = P / (k_B T): its tangent
+    const double Tdc = Td ? Td[cell] : 0.0, Pdc = Pd ? Pd[cell] : 0.0;
+    const double ndd = nd * (Pdc / Pa[cell] - Tdc / Tc);
+    if (st == 0) {
+      cell_rates(ws, rk, Tc, nd, net, Jc, lane);
+      rates_tangent(ts, rkd, Tc, Tdc, nd, ndd, net, (NJ && Jd) ? Jd + (size_t)cell * NJ : nullptr, lane);
+    }
+    const double hin = hio[cell];
+    const double h0 = isfinite(hin) ? hin : dt;
+    if (lane == 0 && !isfinite(hin)) atomicAdd(count + COUNT_NONFINITE_H, 1);
+    Step s = {fmax(fmin(h0, dt), H_MIN), 0.0, 0.0, 0.0, 0.0, false};
+    int it = (dt > 0.0 && st == 0) ? 0 : max_steps;
+    int ntry = 0;
+    while (it < max_steps && s.t < dt * CALL_END) {
+      const double ha = fmin(s.h, dt - s.t);
+      double K1l, K1t, K3l, K3t, K4l, K4t;
+      StageKeep kp;
+      double dtr;
+      stages(c, a21, Nl, Nt, ha, rtol, atol, net, nd, ws, rk, lane, K1l, K1t, K3l, K3t, K4l, K4t, &kp, &dtr);
+      bool ok = false;
+      double Yl, Yt;
+      const double en = try_error(Nl, Nt, K1l, K1t, K3l, K3t, K4l, K4t, Yl, Yt, s.t == 0.0, rtol, atol, lane, ok);
+      if (ok) {
+        // the tangent of the accepted try, with its factors still in place
+        const double ih = 1.0 / ha;
+        double Fel, Fet, Gl, Gt, Ql, Qt;  // Fdot(N); Jdot K_i (eps-del parts)
+        double K1dl, K1dt, K2dl, K2dt, K3dl, K3dt, K4dl, K4dt;
+        rhs_tangent<true>(Nl, Nt, Ndl, Ndt, K1l, K1t, Fel, Fet, Gl, Gt, net, nd, ndd, ts, rk, rkd, lane);
+        K1dl = Fel + Gl;
+        K1dt = Fet + Gt;
+        w_solve(K1dl, K1dt, c, a21, kp, dtr, ws, lane);
+        rhs_tangent<true>(Nl, Nt, Ndl, Ndt, kp.K2l, kp.K2t, Ql, Qt, Gl, Gt, net, nd, ndd, ts, rk, rkd, lane);
+        K2dl = fma(RODAS_C21 * ih, K1dl, Fel) + Gl;
+        K2dt = fma(RODAS_C21 * ih, K1dt, Fet) + Gt;
+        w_solve(K2dl, K2dt, c, a21, kp, dtr, ws, lane);
+        // stage 3: F at Y3 = N + 2 K1 along Ydot3 = Ndot + 2 Kdot1; Jdot K3 at N
+        double Y3el = Ndl + 2.0 * K1dl, Y3et = Ndt + 2.0 * K1dt, dummyl, dummyt;
+        rhs_tangent<false>(Nl + 2.0 * K1l, Nt + 2.0 * K1t, Y3el, Y3et, 0.0, 0.0, Ql, Qt, dummyl, dummyt, net, nd,
+                           ndd, ts, rk, rkd, lane);
+        rhs_tangent<true>(Nl, Nt, Ndl, Ndt, K3l, K3t, dummyl, dummyt, Gl, Gt, net, nd, ndd, ts, rk, rkd, lane);
+        K3dl = fma(ih, K1dl - K2dl, Ql) + Gl;
+        K3dt = fma(ih, K1dt - K2dt, Qt) + Gt;
+        w_solve(K3dl, K3dt, c, a21, kp, dtr, ws, lane);
+        // stage 4: F at Y4 = N + 2 K1 + K3 along Ndot + 2 Kdot1 + Kdot3; Jdot K4 at N
+        rhs_tangent<false>(Nl + 2.0 * K1l + K3l, Nt + 2.0 * K1t + K3t, Y3el + K3dl, Y3et + K3dt, 0.0, 0.0, Ql, Qt,
+                           dummyl, dummyt, net, nd, ndd, ts, rk, rkd, lane);
+        rhs_tangent<true>(Nl, Nt, Ndl, Ndt, K4l, K4t, dummyl, dummyt, Gl, Gt, net, nd, ndd, ts, rk, rkd, lane);
+        K4dl = fma(ih, fma(-RODAS_C43, K3dl, K1dl - K2dl), Ql) + Gl;
+        K4dt = fma(ih, fma(-RODAS_C43, K3dt, K1dt - K2dt), Qt) + Gt;
+        w_solve(K4dl, K4dt, c, a21, kp, dtr, ws, lane);
+        // the new state's tangent: zero where the state is clamped at 0
+        const double ydl = Ndl + 2.0 * K1dl + K3dl + K4dl, ydt = Ndt + 2.0 * K1dt + K3dt + K4dt;
+        Ndl = lane < K0 && Yl >= 0.0 ? ydl : 0.0;
+        Ndt = lane < NT && Yt >= 0.0 ? ydt : 0.0;
+        Nl = fmax(Yl, 0.0);
+        Nt = fmax(Yt, 0.0);
+        s.t += ha;
+        s.nok += 1.0;
+      }
+      const double fac = step_factor(en, ok, s.prej, s.pen, s.pha, ha);
+      s.h = ha * fac;
+      s.prej = !ok;
+      s.pen = en;
+      s.pha = ha;
+      it++;
+      ntry++;
+    }
+    const bool fin = st == 0 && s.t >= dt * CALL_END;
+    double* xo = xout + (size_t)cell * NS;
+    double* xdo = xd + (size_t)cell * NS;
+    int code = 0;
+    if (st == 0)
+      code = __builtin_amdgcn_readfirstlane(guard_wave(Nl, Nt, xr, xo, fin, s.nok, atol, net, lane));
+    else
+      write_state(Nl, Nt, xo, net, lane);
+    if (st == 0 && code == 0) {
+      guard_tangent(Nl, Nt, Ndl, Ndt, xr, xdr, xdo, net, lane);
+    } else {
+      for (int i = lane; i < NS; i += WAVE) xdo[i] = DNAN;
+    }
+    if (lane == 0) {
+      hio[cell] = s.h;
+      doneo[cell] = fin ? 1 : 0;
+      noko[cell] = s.nok;
+      if (tries) tries[cell] = (double)ntry;
+      if (st != 0) {
+        atomicAdd(count + (st == REFUSED_COLD ? COUNT_COLD : st == REFUSED_HOT ? COUNT_HOT : COUNT_BAD_INPUT), 1);
+      } else if (code != 0) {
+        if (code & GUARD_NONFINITE) atomicAdd(count + COUNT_NONFINITE, 1);
+        if (code & GUARD_UNFINISHED) atomicAdd(count + COUNT_UNFINISHED, 1);
+        if (code & GUARD_DRIFT) atomicAdd(count + COUNT_DRIFT, 1);
+        list[atomicAdd(count + COUNT_RECOMPUTED, 1)] = cell;
+      }
+      redo[cell] = code != 0 ? 1 : 0;
+    }
+    int nx = 0;
+    if (lane == 0) nx = waves + atomicAdd(count + COUNT_QUEUE, 1);
+    q = __builtin_amdgcn_readfirstlane(nx);
+  }
+}

@@ -24,7 +24,8 @@ __device__ __forceinline__ void factor_trailing32(float (&c)[NT], float& dt, dou
     if (lane != k) c[k] *= dk;
     const float l = lane > k ? c[k] : 0.0f;
 #pragma unroll
-    for (int j = k + 1; j < NT; j++) c[j] = fmaf(-l, readlane_f(c[j], k), c[j]);
+    for (int j = k + 1; j < NT; j++)
+      if ((TRAIL_FILL[k] >> j) & 1ull) c[j] = fmaf(-l, readlane_f(c[j], k), c[j]);  // structural zeros left out
   }
   wave_sync();
   dt = lane < NT ? dinv[lane] : 1.0f;

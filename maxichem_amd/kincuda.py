@@ -77,6 +77,8 @@ class CudaSolver(Solver):
         guard and recovery stay FP64.
     """
 
+    _EXTRA_KERNELS: tuple[str, ...] = ()  # kernel files compiled into the module after the call's (a subclass's)
+
     def __init__(self, net, device, *, drift: float | None = None, photo: bool = False,
                  variant: tuple[int, ...] | None = None, tune: bool = False, precision: str = "fp64") -> None:  # fmt: skip
         if precision not in ("fp64", "mixed"):
@@ -113,7 +115,8 @@ class CudaSolver(Solver):
                     f"one cell's {what} kernel needs {need} B of LDS per workgroup; the device has "
                     f"{info['lds_per_block']}"
                 )
-        self._source = _kernels.source(tab, self.cpw, self.drift, self.batch, mixed, self.wps, bool(self.rates_global))
+        self._source = _kernels.source(tab, self.cpw, self.drift, self.batch, mixed, self.wps, bool(self.rates_global),
+                                       self._EXTRA_KERNELS)  # fmt: skip
         self._code = code = _hip.compile_source(self._source)
         self._module = _hip.Module(code, dev)
         self._call = self._module.function("mxc_call")

@@ -1,0 +1,141 @@
+"""The tangent (JVP) of one solver call on the GPU: ADSolver, with the reference's call form (maxichem/kincuda_ad.py:
+ADSolver.jvp, JvpResult; docs/method.md, "Derivatives").
+
+Built: the frozen contract (adapt=False): the accepted step sizes held fixed, rejected tries dropped, no tangent
+through dt or h, and none of h out. The tangents of x, T, P and J enter; the call's result is CudaSolver.launch's,
+bit for bit (the same kernels' arithmetic in the same order). A cell the guard fails is recomputed by the recovery
+without a tangent: its tangent is NaN, its flag 1 or 2 as in launch.
+
+Not built (refused by name): the adaptive contract (adapt=True), the rate multipliers lm and the rate table rc with
+their tangents, and the adjoint (vjp).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import NamedTuple
+
+import torch
+
+from . import _hip, _kernels
+from ._constants import ATOL, DEFAULT_MAX_STEPS, DEFAULT_RTOL, REC_STEPS, WAVE
+from ._solver import check_inputs, check_photolysis, check_tensor
+from .kincuda import CudaSolver
+
+__all__ = ["ADSolver", "JvpResult"]
+DMAX = 64  # directions of one jvp call (the reference's bound on the leading axis)
+
+
+class JvpResult(NamedTuple):
+    """What ADSolver.jvp returns, per cell (the reference's JvpResult): x, h, done, steps as CudaSolver's call;
+    dx [B, S] or [D, B, S] the tangent of x; dh [B] or [D, B] the tangent of h (zero under the frozen contract)."""
+
+    x: torch.Tensor
+    h: torch.Tensor
+    done: torch.Tensor
+    steps: torch.Tensor
+    dx: torch.Tensor
+    dh: torch.Tensor
+
+
+class ADSolver(CudaSolver):
+    """One solver call with its tangent on one GPU (the reference's ADSolver(net, device, *, cap, drift, photo)).
+
+    cap, the reference's stored attempts of its adjoint, is accepted and checked (an integer from 1) for the same
+    construction; the tangent does not store attempts."""
+
+    _EXTRA_KERNELS = ("jvp.hip",)
+
+    def __init__(self, net, device, *, cap: int = 64, drift: float | None = None, photo: bool = False) -> None:
+        if not isinstance(cap, int) or isinstance(cap, bool):
+            raise ValueError(f"cap: an integer expected, got {cap!r}")
+        if cap < 1:
+            raise ValueError(f"cap: at least 1 stored attempt expected, got {cap}")
+        super().__init__(net, device, drift=drift, photo=photo, variant=(1, _kernels.DEFAULT_BATCH, 1, 0))
+        self.cap = cap
+        with torch.cuda.device(self.dev):
+            self._jvp = self._module.function("mxc_jvp")
+            self.lds_jvp = _kernels.jvp_lds_bytes(self.tab, 1)
+            info = _hip.device_info(self.dev.index)
+            if self.lds_jvp > info["lds_per_block"]:
+                raise ValueError(f"one cell's tangent kernel needs {self.lds_jvp} B of LDS; the device has "
+                                 f"{info['lds_per_block']}")  # fmt: skip
+            self.max_blocks_jvp = info["compute_units"] * max(1, _hip.occupancy(self._jvp, WAVE, self.lds_jvp))
+
+    def jvp(self, x, T, P, dt, h, dx, dT, dP, ddt, dh, *, lm=None, dlm=None, rc=None, drc=None, adapt: bool = True,
+            rtol: float = DEFAULT_RTOL, atol: float = ATOL, max_steps: int = DEFAULT_MAX_STEPS, J=None,
+            dJ=None) -> JvpResult:  # fmt: skip
+        """The call and its tangent (frozen contract: pass adapt=False). Everything after dh is keyword only.
+
+        dx, dT, dP, ddt, dh: the inputs' tangents in their shapes, or with a leading axis of D <= 64 directions on
+        every one; dJ [B, branches] (or [D, B, branches]) with photolysis. The result's x, h, done and steps are
+        CudaSolver.launch's; dx the tangent of x; dh zero. Refusals and counts as launch's."""
+        self.flag = None
+        self._require_open()
+        for name, a in (("lm", lm), ("dlm", dlm), ("rc", rc), ("drc", drc)):
+            if a is not None:
+                raise NotImplementedError(f"{name}: rate multipliers and the rate table are not differentiated here")
+        if adapt:
+            raise NotImplementedError("adapt=True: only the frozen contract (adapt=False) is built")
+        B = check_inputs(self.dev, self.ns, x, T, P, dt, h, rtol, atol, max_steps)
+        if dx.dim() not in (2, 3):
+            raise ValueError(f"dx: [B, S] or [D, B, S] expected, got {list(dx.shape)}")
+        one = dx.dim() == 2
+        D = 1 if one else dx.shape[0]
+        if not 1 <= D <= DMAX:
+            raise ValueError(f"dx: from 1 to {DMAX} directions on the leading axis expected, got {list(dx.shape)}")
+        lead = () if one else (D,)
+        if dJ is not None and not self.nj and (dJ.dim() != len(lead) + 2 or dJ.shape[-1] != 0):
+            raise ValueError(f"dJ: none or an empty {[*lead, B, 0]} one expected, since this solver has no photolysis; "
+                             f"got {list(dJ.shape)}")  # fmt: skip
+        shapes = {"dx": (*lead, B, self.ns), "dT": (*lead, B), "dP": (*lead, B), "ddt": (*lead, B), "dh": (*lead, B),
+                  "dJ": (*lead, B, self.nj)}  # fmt: skip
+        for name, a in (("dx", dx), ("dT", dT), ("dP", dP), ("ddt", ddt), ("dh", dh), ("dJ", dJ)):
+            if a is not None:
+                check_tensor(name, a, shapes[name], self.dev)
+        with torch.cuda.device(self.dev):
+            x, T, P, dt, h = (a.contiguous() for a in (x, T, P, dt, h))
+            jx = check_photolysis(J, B, self.nj, self.dev)
+            dJ = dJ if self.nj else None
+            tx = dx if not one else dx[None]
+            tT, tP = (None if a is None else (a if not one else a[None]) for a in (dT, dP))
+            tJ = None if dJ is None else (dJ if not one else dJ[None])
+            xd = torch.empty(D, B, self.ns, dtype=torch.float64, device=self.dev)
+            result = None
+            for d in range(D):  # each direction repeats the call, with the same bits
+                result = self._jvp_once(x, T, P, dt, h, jx, rtol, atol, max_steps, B, tx[d].contiguous(),
+                                        None if tT is None else tT[d].contiguous(),
+                                        None if tP is None else tP[d].contiguous(),
+                                        None if tJ is None else tJ[d].contiguous(), xd[d])  # fmt: skip
+            xo, ho, done, nok, redo = result
+            self.total += self.count
+            self._refuse(x, T, P, dt, jx, self._status)
+        self.flag = redo
+        dxo = xd[0] if one else xd
+        dho = torch.zeros((*lead, B), dtype=torch.float64, device=self.dev)
+        return JvpResult(xo, ho, done.bool(), nok, dxo, dho)
+
+    def _jvp_once(self, x, T, P, dt, h, jx, rtol, atol, max_steps, B, dx, dT, dP, dJ, xd):
+        xo, ho = torch.empty_like(x), h.clone()
+        done, redo, lst, status = (torch.empty(B, dtype=torch.int32, device=self.dev) for _ in range(4))
+        nok = torch.empty(B, dtype=torch.float64, device=self.dev)
+        self.tries = torch.zeros(B, dtype=torch.float64, device=self.dev)
+        self.count.zero_()
+        p = _hip.ptr
+        stream = torch.cuda.current_stream(self.dev).cuda_stream
+        args = [p(self._net), p(x), p(T), p(P), p(dt), p(ho), p(xo), p(done), p(nok), p(status), p(redo), p(lst),
+                p(self.count), p(self.tries), p(jx), ctypes.c_int(B), ctypes.c_double(rtol), ctypes.c_double(atol),
+                ctypes.c_int(max_steps), p(dx), p(dT), p(dP), p(dJ), p(xd)]  # fmt: skip
+        if B:
+            _hip.launch(self._jvp, max(1, min(B, self.max_blocks_jvp)), WAVE, self.lds_jvp, stream, args)
+        # the cells the guard failed: the recovery's result, without a tangent (NaN from the tangent kernel)
+        rec_args = [p(self._net), p(x), p(T), p(P), p(dt), p(h), p(xo), p(ho), p(done), p(nok), p(redo), p(lst),
+                    p(self.count), p(self.tries), p(jx), ctypes.c_double(rtol), ctypes.c_double(atol),
+                    ctypes.c_int(REC_STEPS * max_steps), p(self._rscratch)]  # fmt: skip
+        _hip.launch(self._recover, max(1, min(B, self.rec_blocks)), WAVE * _kernels.REC_WAVES, self.lds_rec, stream,
+                    rec_args)  # fmt: skip
+        self._status = status
+        return xo, ho, done, nok, redo
+
+    def vjp(self, *args, **kwargs):
+        raise NotImplementedError("the adjoint (vjp) is not built here; the tangent (jvp, adapt=False) is")

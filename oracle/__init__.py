@@ -142,6 +142,37 @@ class Oracle:
         )  # fmt: skip
         return {"h": h.value, "t": t.value, "steps": nok.value, "tries": tries.value}
 
+    def accepted_steps(self, x0, T: float, P: float, dt: float, h0: float, *, rtol=1e-3, atol=1e-25,
+                       max_steps=4000, J=None) -> np.ndarray:  # fmt: skip
+        """The sizes of one cell's accepted tries in a call (the call kernel's tries, without the guard)."""
+        x0 = np.ascontiguousarray(x0, np.float64)
+        Jp = None
+        if self.tab.nj:
+            J = np.zeros(self.tab.nj) if J is None else np.ascontiguousarray(J, np.float64)
+            Jp = _p(J, _d)
+        steps = np.empty(max_steps)
+        n = lib().oracle_accepted_steps(
+            ctypes.byref(self.net), _p(x0, _d), ctypes.c_double(T), ctypes.c_double(P), Jp, ctypes.c_double(dt),
+            ctypes.c_double(h0), ctypes.c_double(rtol), ctypes.c_double(atol), ctypes.c_int(max_steps),
+            _p(steps, _d), ctypes.c_int(max_steps),
+        )  # fmt: skip
+        return steps[:n].copy()
+
+    def replay(self, x0, T: float, P: float, steps, *, atol=1e-25, J=None) -> tuple[int, np.ndarray]:
+        """The frozen primal: from x0, the given tries each accepted, then the guard (bits, x out)."""
+        x0 = np.ascontiguousarray(x0, np.float64)
+        steps = np.ascontiguousarray(steps, np.float64)
+        Jp = None
+        if self.tab.nj:
+            J = np.zeros(self.tab.nj) if J is None else np.ascontiguousarray(J, np.float64)
+            Jp = _p(J, _d)
+        out = np.empty(self.tab.ns)
+        code = lib().oracle_replay(
+            ctypes.byref(self.net), _p(x0, _d), ctypes.c_double(T), ctypes.c_double(P), Jp, _p(steps, _d),
+            ctypes.c_int(len(steps)), ctypes.c_double(atol), _p(out, _d),
+        )  # fmt: skip
+        return code, out
+
     def check_cell(self, x0, xout, done: bool, nok: float, atol: float = 1e-25) -> tuple[int, np.ndarray]:
         x0 = np.ascontiguousarray(x0, np.float64)
         xo = np.array(xout, np.float64)

@@ -405,14 +405,26 @@ static double try_error(int ns, const double* N, const double* K1, const double*
 /* ---- one pass of Rodas3 tries from the state N: the call kernel (dense = 0) or the recovery (dense = 1) ---- */
 
 /* Runs the try loop; returns the step state. N (elimination order) is advanced in place; *tries counts. */
-static step_t try_loop(cell_t* c, const int* inv, double h0, double dt, int max_steps, double rtol, double atol,
-                       int dense, double* tries) {
+/* Test hooks: forced (not NULL) replays exactly nforced tries of the given sizes, each accepted whatever its error
+ * (the frozen primal of a tangent); record (not NULL) receives the accepted tries' sizes, at most cap of them, and
+ * *nrecord their number. */
+typedef struct {
+  const double* forced;
+  int nforced;
+  double* record;
+  int cap;
+  int* nrecord;
+} try_hooks;
+
+static step_t try_loop_hooked(cell_t* c, const int* inv, double h0, double dt, int max_steps, double rtol,
+                              double atol, int dense, double* tries, const try_hooks* hk) {
   const int ns = c->ns;
   step_t s = {h0, 0.0, 0.0, 0.0, 0.0, 0};
-  int it = dt > 0.0 ? 0 : max_steps;
+  const int forced = hk && hk->forced;
+  int it = (dt > 0.0 || forced) ? 0 : max_steps;
   double* W = c->A;
-  while (it < max_steps && s.t < dt * CALL_END) {
-    const double ha = fmin(s.h, dt - s.t), igh = 1.0 / (GAMMA * ha), ih = 1.0 / ha;
+  while (forced ? it < hk->nforced : (it < max_steps && s.t < dt * CALL_END)) {
+    const double ha = forced ? hk->forced[it] : fmin(s.h, dt - s.t), igh = 1.0 / (GAMMA * ha), ih = 1.0 / ha;
     /* a retry at the same state recomputes F, u and -Jn: the same inputs give the same values */
     const double ntot = rhs(c, c->N, c->F, c->u);
     minus_jn(c, inv);
@@ -477,10 +489,12 @@ static step_t try_loop(cell_t* c, const int* inv, double h0, double dt, int max_
     }
     int ok = 0;
     const double en = try_error(ns, c->N, c->K1, c->K3, c->K4, c->Y, s.t == 0.0, rtol, atol, &ok);
+    if (forced) ok = 1;
     if (ok) { /* accept_try (common.cuh:111-118) */
       for (int i = 0; i < ns; i++) c->N[i] = fmax(c->Y[i], 0.0);
       s.t += ha;
       s.nok += 1.0;
+      if (hk && hk->record && *hk->nrecord < hk->cap) hk->record[(*hk->nrecord)++] = ha;
     }
     const double fac = step_factor(en, ok, s.prej, s.pen, s.pha, ha);
     s.h = ha * fac; /* next_try (common.cuh:128-133) */
@@ -491,6 +505,11 @@ static step_t try_loop(cell_t* c, const int* inv, double h0, double dt, int max_
     *tries += 1.0;
   }
   return s;
+}
+
+static step_t try_loop(cell_t* c, const int* inv, double h0, double dt, int max_steps, double rtol, double atol,
+                       int dense, double* tries) {
+  return try_loop_hooked(c, inv, h0, dt, max_steps, rtol, atol, dense, tries, NULL);
 }
 
 /* x = N / sum N in network order (common.cuh:592-599). */
@@ -683,6 +702,50 @@ double oracle_step_factor(double en, int ok, int prej, double pen, double pha, d
 /* The tries of one cell without the guard and the recovery: the call kernel's part of one_cell (dense 0), or one pass
  * of the recovery (dense 1: the dense W with threshold pivoting), from x0 (network order) at h0 over dt, at most
  * max_steps tries. Outputs the next step size, the time reached, the accepted steps and the tries. */
+/* A cell set up for one test run: rates at (T, P, J), N = x0 in elimination order. */
+static void test_cell(cell_t* c, const oracle_net* net, net_column* col, int* inv, const double* x0, double T,
+                      double P, const double* J) {
+  const int ns = net->ns;
+  cell_alloc(c, net, col);
+  c->nd = P / (KB * T);
+  oracle_rates(net, T, P, J, c->kf, c->kr);
+  for (int k = 0; k < ns; k++) inv[net->perm[k]] = k;
+  for (int i = 0; i < ns; i++) c->N[i] = x0[net->perm[i]];
+}
+
+int oracle_accepted_steps(const oracle_net* net, const double* x0, double T, double P, const double* J, double dt,
+                          double h0, double rtol, double atol, int max_steps, double* steps, int cap) {
+  net_column* col = net_columns(net);
+  cell_t c;
+  int* inv = (int*)calloc((size_t)net->ns, sizeof(int));
+  test_cell(&c, net, col, inv, x0, T, P, J);
+  int n = 0;
+  double tries = 0.0;
+  const try_hooks hk = {NULL, 0, steps, cap, &n};
+  (void)try_loop_hooked(&c, inv, fmax(fmin(h0, dt), H_MIN), dt, max_steps, rtol, atol, 0, &tries, &hk);
+  free(inv);
+  free(col);
+  cell_free(&c);
+  return n;
+}
+
+int oracle_replay(const oracle_net* net, const double* x0, double T, double P, const double* J, const double* steps,
+                  int n, double atol, double* xout) {
+  net_column* col = net_columns(net);
+  cell_t c;
+  int* inv = (int*)calloc((size_t)net->ns, sizeof(int));
+  test_cell(&c, net, col, inv, x0, T, P, J);
+  double tries = 0.0;
+  const try_hooks hk = {steps, n, NULL, 0, NULL};
+  const step_t s = try_loop_hooked(&c, inv, steps[0], 1.0, n, 1.0, atol, 0, &tries, &hk);
+  write_state(&c, xout);
+  const int code = oracle_check_cell(net, x0, xout, 1, s.nok, atol);
+  free(inv);
+  free(col);
+  cell_free(&c);
+  return code;
+}
+
 void oracle_try_loop(const oracle_net* net, const double* x0, double T, double P, const double* J, double dt,
                      double h0, double rtol, double atol, int max_steps, int dense, double* h, double* t, double* nok,
                      double* tries) {

@@ -60,6 +60,16 @@ void oracle_try_loop(const oracle_net* net, const double* x0, double T, double P
                      double h0, double rtol, double atol, int max_steps, int dense, double* h, double* t, double* nok,
                      double* tries);
 
+/* The frozen primal of a tangent (test hooks). oracle_accepted_steps: one cell's call tries as oracle_try_loop takes
+ * them (dense 0), recording the accepted tries' sizes into steps (at most cap); returns how many were accepted.
+ * oracle_replay: from x0 at (T, P, J), exactly n tries of the sizes steps[], each accepted whatever its error, then
+ * x = N / sum N and the guard's check and correction (done, nok n) into xout (network order); returns the guard's
+ * bits. */
+int oracle_accepted_steps(const oracle_net* net, const double* x0, double T, double P, const double* J, double dt,
+                          double h0, double rtol, double atol, int max_steps, double* steps, int cap);
+int oracle_replay(const oracle_net* net, const double* x0, double T, double P, const double* J, const double* steps,
+                  int n, double atol, double* xout);
+
 /* The guard's test and share correction of one result (check_cell): returns the GUARD_* bits, 0 when it passes, in
  * which case xout is corrected in place. */
 int oracle_check_cell(const oracle_net* net, const double* x0, double* xout, int done, double nok, double atol);

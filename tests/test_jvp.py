@@ -1,0 +1,131 @@
+"""The tangent of a call under the frozen contract (ADSolver.jvp, adapt=False), against central differences of the
+oracle's frozen primal: the cell's accepted tries replayed at the same sizes from perturbed inputs (the reference
+checks its tangent the same way, maxichem tests/test_jvp.py, and its primal bitwise against the call,
+tests/test_kernel_copies.py).
+
+- CPU: the oracle's replay of a cell's accepted tries gives the call's result bit for bit, and its central
+  differences converge (two step sizes agree), so they pin the tangent.
+- GPU: jvp's x, h, done and steps are CudaSolver.launch's bit for bit; its tangent along random directions of x, T,
+  P and J matches the central differences to 1e-5 of each cell's largest tangent entry; several directions in one
+  call give each direction's tangent; the refusals of what is not built.
+"""
+
+import numpy as np
+import oracle
+import pytest
+import torch
+
+from maxichem_amd import _tables, standin, workloads
+
+CELLS = 48
+EPS = 1e-6  # the central differences' step along a direction (directions scaled to relative moves of 1e-2)
+TANGENT_TOLERANCE = 1e-5  # of a cell's largest tangent entry: above the differences' own error (about 1e-6)
+
+
+def case(seed: int = 5):
+    """NCHO cells relaxed by the oracle (dt 1e8 s, dark), then called for dt 300 s, a quarter lit; random
+    directions: x moved relatively with its sum kept, T and P by up to 0.5 %, J by up to 50 %."""
+    net = standin.load("NCHO")
+    tab = _tables.build(net, True)
+    orc = oracle.Oracle(tab)
+    inp = workloads.config1(net.species, len(net.photo), cells=CELLS, seed=seed)
+    r = orc.call(inp["x"], inp["T"], inp["P"], inp["dt_relax"], inp["h"], J=np.zeros_like(inp["J"]))
+    rng = np.random.default_rng(seed)
+    x = r["x"]
+    dx = x * rng.uniform(-1.0, 1.0, x.shape)
+    dx -= x * (dx.sum(1, keepdims=True) / x.sum(1, keepdims=True))
+    d = {"dx": dx, "dT": 0.005 * inp["T"] * rng.uniform(-1, 1, CELLS), "dP": 0.005 * inp["P"] * rng.uniform(-1, 1, CELLS),
+         "dJ": 0.5 * inp["J"] * rng.uniform(-1, 1, inp["J"].shape)}  # fmt: skip
+    return net, orc, {"x": x, "T": inp["T"], "P": inp["P"], "dt": inp["dt"], "h": r["h"], "J": inp["J"]}, d
+
+
+def frozen_differences(orc, inp, d, b):
+    """Central differences of cell b's frozen primal along d (None when the oracle's guard fails the cell)."""
+    steps = orc.accepted_steps(inp["x"][b], inp["T"][b], inp["P"][b], inp["dt"][b], inp["h"][b], J=inp["J"][b])
+
+    def f(e):
+        return orc.replay(inp["x"][b] + e * d["dx"][b], inp["T"][b] + e * d["dT"][b], inp["P"][b] + e * d["dP"][b],
+                          steps, J=inp["J"][b] + e * d["dJ"][b])  # fmt: skip
+
+    (cp, up), (cm, down) = f(EPS), f(-EPS)
+    return None if cp or cm else (up - down) / (2.0 * EPS), len(steps)
+
+
+def test_replay_is_the_call_and_its_differences_converge():
+    net, orc, inp, d = case()
+    out = orc.call(inp["x"], inp["T"], inp["P"], inp["dt"], inp["h"], J=inp["J"])
+    for b in range(0, CELLS, 6):
+        steps = orc.accepted_steps(inp["x"][b], inp["T"][b], inp["P"][b], inp["dt"][b], inp["h"][b], J=inp["J"][b])
+        code, x = orc.replay(inp["x"][b], inp["T"][b], inp["P"][b], steps, J=inp["J"][b])
+        assert code == 0 and out["flag"][b] == 0 and np.array_equal(x, out["x"][b]), b
+        assert len(steps) == out["steps"][b]
+        fd, _ = frozen_differences(orc, inp, d, b)
+        coarse = {}
+        for e in (1e-5, 1e-7):
+            up = orc.replay(inp["x"][b] + e * d["dx"][b], inp["T"][b] + e * d["dT"][b], inp["P"][b] + e * d["dP"][b],
+                            steps, J=inp["J"][b] + e * d["dJ"][b])[1]  # fmt: skip
+            down = orc.replay(inp["x"][b] - e * d["dx"][b], inp["T"][b] - e * d["dT"][b], inp["P"][b] - e * d["dP"][b],
+                              steps, J=inp["J"][b] - e * d["dJ"][b])[1]  # fmt: skip
+            coarse[e] = np.abs((up - down) / (2 * e) - fd).max() / np.abs(fd).max()
+        print(f"cell {b}: {len(steps)} accepted tries, differences at 1e-5 and 1e-7 against 1e-6: {coarse}")
+        assert coarse[1e-5] <= 1e-6 and coarse[1e-7] <= 1e-5  # truncation, then rounding, both below the bar
+
+
+def t(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.gpu
+def test_jvp_repeats_the_call_and_matches_the_frozen_differences(gpu):
+    from maxichem_amd.kincuda import CudaSolver
+    from maxichem_amd.kincuda_ad import ADSolver
+
+    net, orc, inp, d = case()
+    args = [t(inp[k]) for k in ("x", "T", "P", "dt", "h")]
+    with CudaSolver(net, "cuda", photo=True) as plain:
+        want = plain.launch(*args, J=t(inp["J"]))
+    with ADSolver(net, "cuda", photo=True) as ad:
+        r = ad.jvp(*args, t(d["dx"]), t(d["dT"]), t(d["dP"]), t(np.zeros(CELLS)), t(np.zeros(CELLS)), adapt=False,
+                   J=t(inp["J"]), dJ=t(d["dJ"]))  # fmt: skip
+        flag = ad.flag.cpu().numpy()
+    for name in ("x", "h", "steps"):
+        assert torch.equal(getattr(r, name), getattr(want, name)), name
+    assert torch.equal(r.done, want.done) and np.array_equal(flag, want.flag.cpu().numpy())
+    assert not bool(r.dh.any())
+    dxg = r.dx.cpu().numpy()
+    worst, compared = 0.0, 0
+    for b in range(CELLS):
+        fd, n = frozen_differences(orc, inp, d, b)
+        if fd is None or flag[b] != 0 or n != int(want.steps[b]):
+            continue  # the guard failed, or the oracle's accepted tries are not the GPU's
+        err = np.abs(dxg[b] - fd).max() / np.abs(fd).max()
+        worst = max(worst, err)
+        compared += 1
+    print(f"{compared} of {CELLS} cells compared; largest tangent error against the differences {worst:.2e}")
+    assert compared >= CELLS * 3 // 4 and worst <= TANGENT_TOLERANCE
+
+
+@pytest.mark.gpu
+def test_directions_share_one_call_and_unbuilt_parts_are_refused(gpu):
+    from maxichem_amd.kincuda_ad import ADSolver
+
+    net, _, inp, d = case(seed=7)
+    args = [t(inp[k]) for k in ("x", "T", "P", "dt", "h")]
+    zero = t(np.zeros(CELLS))
+    with ADSolver(net, "cuda", photo=True) as ad:
+        one = [ad.jvp(*args, t(s * d["dx"]), t(s * d["dT"]), t(s * d["dP"]), zero, zero, adapt=False,
+                      J=t(inp["J"]), dJ=t(s * d["dJ"])).dx for s in (1.0, -2.0)]  # fmt: skip
+        stack = lambda a: t(np.stack([a, -2.0 * a]))  # noqa: E731
+        both = ad.jvp(*args, stack(d["dx"]), stack(d["dT"]), stack(d["dP"]), stack(np.zeros(CELLS)),
+                      stack(np.zeros(CELLS)), adapt=False, J=t(inp["J"]), dJ=stack(d["dJ"]))  # fmt: skip
+        assert both.dx.shape == (2, CELLS, len(net.species)) and both.dh.shape == (2, CELLS)
+        assert torch.equal(both.dx[0], one[0]) and torch.equal(both.dx[1], one[1])
+        ok = torch.isfinite(one[0]).all(1)
+        assert torch.allclose(one[1][ok], -2.0 * one[0][ok], rtol=1e-9, atol=0.0)  # linear in the direction
+        with pytest.raises(NotImplementedError, match="adapt=True"):
+            ad.jvp(*args, t(d["dx"]), zero, zero, zero, zero, J=t(inp["J"]))
+        with pytest.raises(NotImplementedError, match="lm"):
+            ad.jvp(*args, t(d["dx"]), zero, zero, zero, zero, adapt=False, J=t(inp["J"]),
+                   lm=torch.zeros(CELLS, ad.nr, dtype=torch.float64, device="cuda"))  # fmt: skip
+        with pytest.raises(ValueError, match="dx"):
+            ad.jvp(*args, t(d["dx"][:, :5]), zero, zero, zero, zero, adapt=False, J=t(inp["J"]))

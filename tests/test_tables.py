@@ -145,3 +145,40 @@ def test_tables_refuse_networks_beyond_the_kernels_limits(read_network):
     S[0, 0] = 3.0
     with pytest.raises(ValueError, match=r"\|S\| <= 2 per species and reaction"):
         _tables.build(types.SimpleNamespace(**dict(fields, S=S)))
+
+
+def test_trailing_fill_is_the_symbolic_lu_of_minus_jn(read_network):
+    # the trailing rows' filled pattern the factorization reads, against an elimination written here on the pattern of
+    # -Jn (a species whose net stoichiometry a reaction changes, against each species the reaction's rate depends on)
+    # plus the diagonal; and a dense numerical LU of a random matrix on that pattern is zero wherever it has no entry
+    for net, photo in _nets(read_network):
+        tab = _tables.build(net, photo)
+        ns, k0 = tab.ns, tab.k0
+        pos = np.argsort(tab.perm)
+        entry = np.eye(ns, dtype=bool)
+        for r in range(tab.nr):
+            x = tab.rx[r]
+            net_change = np.zeros(ns, int)
+            for s_ in x[:3]:
+                if s_ >= 0:
+                    net_change[s_] -= 1
+            for s_ in x[3:]:
+                if s_ >= 0:
+                    net_change[s_] += 1
+            depends = {int(s_) for s_ in (x if r < tab.nrt else x[:3]) if s_ >= 0}
+            for p_ in np.flatnonzero(net_change):
+                for q in depends:
+                    entry[pos[p_], pos[q]] = True
+        want = entry.copy()
+        for k in range(ns):  # fill by elimination, one pivot at a time
+            below = want[k + 1 :, k].nonzero()[0] + k + 1
+            right = want[k, k + 1 :].nonzero()[0] + k + 1
+            want[np.ix_(below, right)] = True
+        got = np.array([[(m >> j) & 1 for j in range(tab.nt)] for m in tab.trail_fill], bool)
+        assert np.array_equal(got, want[k0:, k0:])
+        rng = np.random.default_rng(1)
+        a = np.where(entry, rng.uniform(0.5, 1.5, (ns, ns)), 0.0) + ns * np.eye(ns)
+        for k in range(ns):
+            a[k + 1 :, k] /= a[k, k]
+            a[k + 1 :, k + 1 :] -= np.outer(a[k + 1 :, k], a[k, k + 1 :])
+        assert not a[k0:, k0:][~got].any()

@@ -4,11 +4,12 @@
 //
 // A workgroup of four wavefronts computes one 64 x 64 tile of C = A B. Each step stages a 64 x 32 slice of A and a
 // 32 x 64 slice of B in LDS (both with the inner index contiguous), and each wavefront accumulates its 32 x 32
-// quarter with v_mfma_f32_16x16x16_bf16: lane l holds A rows l % 16 and B columns l % 16 at inner indices
-// 4 (l / 16) .. 4 (l / 16) + 3, and receives C rows 4 (l / 16) .. + 3 of column l % 16.
+// quarter with gfx950's v_mfma_f32_16x16x32_bf16, one product per 16 x 16 block and slice: lane l holds A row l % 16
+// and B column l % 16 at inner indices 8 (l / 16) .. 8 (l / 16) + 7 (16 bytes, one LDS read each), and receives C
+// rows 4 (l / 16) .. + 3 of column l % 16.
 
 typedef unsigned short bf16_t;
-typedef short short4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
 constexpr int TILE_M = 64, TILE_N = 64, TILE_K = 32, LDS_K = TILE_K + 8;  // 8: keeps rows off one bank
@@ -38,8 +39,9 @@ template <int EPI>
 __device__ __forceinline__ void gemm(const bf16_t* __restrict__ A, long sam, long sak, const bf16_t* __restrict__ B,
                                      long sbk, long sbn, int K, int ntn, const float* __restrict__ bias,
                                      float* __restrict__ Cf, bf16_t* __restrict__ Cb, long ldc, float alpha) {
-  __shared__ bf16_t As[TILE_M * LDS_K];
-  __shared__ bf16_t Bs[TILE_N * LDS_K];  // B's slice transposed: column n's inner indices contiguous
+  __shared__ __attribute__((aligned(16))) bf16_t As[TILE_M * LDS_K];
+  __shared__ __attribute__((aligned(16))) bf16_t Bs[TILE_N * LDS_K];  // B's slice transposed: column n's inner
+                                                                        // indices contiguous
   const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6;
   // a one-dimensional grid of ntn tiles per row of tiles
   const long m0 = (long)(blockIdx.x / ntn) * TILE_M, n0 = (long)(blockIdx.x % ntn) * TILE_N;
@@ -58,21 +60,18 @@ __device__ __forceinline__ void gemm(const bf16_t* __restrict__ A, long sam, lon
       Bs[row * LDS_K + kk] = B[(long)(k0 + kk) * sbk + (n0 + row) * sbn];
     }
     __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < TILE_K; ks += 16) {
-      const int kq = ks + 4 * (lane >> 4);
-      short4v a[2], b[2];
+    {
+      const int kq = 8 * (lane >> 4);
+      bf16x8 a[2], b[2];
 #pragma unroll
       for (int i = 0; i < 2; i++) {
-        const bf16_t* pa = As + (wm + 16 * i + (lane & 15)) * LDS_K + kq;
-        const bf16_t* pb = Bs + (wn + 16 * i + (lane & 15)) * LDS_K + kq;
-        a[i] = short4v{(short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3]};
-        b[i] = short4v{(short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3]};
+        a[i] = *(const bf16x8*)(As + (wm + 16 * i + (lane & 15)) * LDS_K + kq);
+        b[i] = *(const bf16x8*)(Bs + (wn + 16 * i + (lane & 15)) * LDS_K + kq);
       }
 #pragma unroll
       for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }

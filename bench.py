@@ -49,6 +49,8 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--variant", default=None,
                     help="the call kernel's variant 'cells per workgroup,batch,waves per SIMD' (default: the tuned or "
                          "the default one)")
+    ap.add_argument("--tune", action="store_true",
+                    help="tune the call kernel's variant on a sample of settled calls first (the calls workload)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the timed calls, copy a tensor of this many MiB once: a kernel of known traffic that "
                          "scales hardware byte counters")
@@ -188,6 +190,12 @@ def train_surrogate(net, plain, args, log_fn):
     return sur, n
 
 
+def solver_drift() -> float:
+    from maxichem_amd._constants import DRIFT
+
+    return DRIFT
+
+
 def run_uneven(args, net, dev) -> int:
     """The uneven batch: one call per step on the same cold inputs, replayed from a captured graph."""
     import numpy as np
@@ -203,9 +211,13 @@ def run_uneven(args, net, dev) -> int:
         return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
 
     x, T, P, dt, h, J = (g(d[k]) for k in ("x", "T", "P", "dt", "h", "J"))
+    from maxichem_amd import tune as _tune
+
     t0 = time.perf_counter()
-    solver = CudaSolver(net, dev, photo=True, tune=True)
-    log(f"autotuned in {time.perf_counter() - t0:.1f} s: variant {solver.info['variant']}")
+    tuned = _tune.tune(net, dev, True, solver_drift(), workload="uneven")
+    solver = CudaSolver(net, dev, photo=True, variant=tuple(tuned["best"]))
+    log(f"autotuned on the uneven sample in {time.perf_counter() - t0:.1f} s: variant {solver.info['variant']}; "
+        f"ms {tuned['ms']}, spilled bytes per lane {tuned['private_bytes']}")  # fmt: skip
     graph = solver.capture(x, T, P, dt, h, J=J)
     graph.replay()  # warm-up: also orders the next replay's queue by this one's tries
     torch.cuda.synchronize()
@@ -222,6 +234,35 @@ def run_uneven(args, net, dev) -> int:
     total = float(np.sum(wall))
     tr = tries[-1]
     work = flops.work(solver.tab)
+    # one eager call with events: the call kernel's and the recovery kernel's shares
+    solver.timing, solver.timing_recover = [], []
+    r = solver.launch(x, T, P, dt, h, J=J)
+    torch.cuda.synchronize()
+    call_ms = float(sum(a.elapsed_time(b) for a, b in solver.timing))
+    rec_ms = float(sum(a.elapsed_time(b) for a, b in solver.timing_recover))
+    solver.timing = None
+    flag = r.flag.cpu().numpy()
+    counts = solver.fallback
+    log(f"eager call: call kernel {call_ms:.1f} ms, recovery {rec_ms:.1f} ms; {counts}")
+    # the oracle on the cells the guard failed and the costliest ones
+    sys.path.insert(0, str(ROOT))
+    import oracle
+
+    from maxichem_amd import _tables
+
+    pick = np.unique(np.concatenate([np.flatnonzero(flag != 0), np.argsort(-tr, kind="stable")[:16]]))
+    ref = oracle.Oracle(_tables.build(net, True)).call(d["x"][pick], d["T"][pick], d["P"][pick], d["dt"][pick],
+                                                       d["h"][pick], J=d["J"][pick])  # fmt: skip
+    xg = r.x.cpu().numpy()[pick]
+    ok = flag[pick] != 2
+    rb = (ref["x"] > 1e-20) & ok[:, None]
+    check = {"cells": int(len(pick)), "recovered_cells": int((flag == 1).sum()), "still_failed_cells":
+             np.flatnonzero(flag == 2).tolist(), "flags_match": bool(np.array_equal(flag[pick], ref["flag"])),
+             "done_match": bool(np.array_equal(r.done.cpu().numpy()[pick], ref["done"])),
+             "tries_match": bool(np.array_equal(solver.tries.cpu().numpy()[pick], ref["tries"])),
+             "max_rel_species_above_1e-20_passed_cells": float(np.abs(xg[rb] / ref["x"][rb] - 1).max()) if rb.any()
+             else 0.0}  # fmt: skip
+    log(f"oracle on the recomputed and costliest cells: {check}")
     achieved = float(np.mean([flops.call_flops(solver.tab, q) for q in tries]) / (total / args.steps) / 1e12)
     line = {
         "metric": "cells/s per call (NCHO, 64K cells): learned+FP32-LU vs plain FP64 at same error",
@@ -233,6 +274,10 @@ def run_uneven(args, net, dev) -> int:
                    "variant": list(solver.info["variant"]), "seed": 3, "form": "plain"},
         "tries_per_cell": {"mean": float(tr.mean()), "p50": float(np.percentile(tr, 50)),
                            "p99": float(np.percentile(tr, 99)), "max": float(tr.max())},
+        "tuning": tuned,
+        "kernel_share": {"call_ms": call_ms, "recovery_ms": rec_ms, "recovery_share": rec_ms / (call_ms + rec_ms),
+                         "counts": counts},
+        "oracle_on_recomputed_and_costliest": check,
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK, "traffic": None, "kernel": "mxc_call (whole replay)",
                      "flop_per_try": work.per_try},
@@ -479,7 +524,12 @@ def main() -> int:
     def g(a):
         return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
 
-    variant = tuple(int(q) for q in args.variant.split(",")) if args.variant else None
+    if args.variant:
+        variant = tuple(int(q) for q in args.variant.split(","))
+    else:  # the autotuner's choice for these settled calls (tuned now with --tune), else the default
+        from maxichem_amd import tune as _tune
+
+        variant = _tune.choose(net, dev, True, solver_drift(), run=args.tune, workload="settled")
     solver = CudaSolver(net, dev, photo=True, variant=variant)
     log(f"solver: {solver.info}")
     x, T, P, h, J = g(d["x"]), g(d["T"]), g(d["P"]), g(d["h"]), g(d["J"])

@@ -135,7 +135,9 @@ class CudaSolver(Solver):
         self.tries: torch.Tensor | None = None
         self._fp64: torch.Tensor | None = None  # the mixed form's cells that fell back to FP64 (_fp64_cells)
         # a list to collect (start, end) events of each call kernel launch on the call's stream; None: not recorded
+        # (when recorded, timing_recover collects the recovery kernel's (start, end) events the same way)
         self.timing: list | None = None
+        self.timing_recover: list = []
         self.info = {
             "ns": tab.ns, "nr": tab.nr, "nj": tab.nj, "k0": tab.k0, "nt": tab.nt, "ndv": tab.ndv,
             "jac_terms": tab.jac_terms, "sv_terms": tab.sv_terms, "lds_call": self.lds_call, "lds_rec": self.lds_rec,
@@ -221,6 +223,10 @@ class CudaSolver(Solver):
                     ctypes.c_int(REC_STEPS * max_steps), p(self._rscratch)]  # fmt: skip
         _hip.launch(self._recover, max(1, min(B, self.rec_blocks)), WAVE * _kernels.REC_WAVES, self.lds_rec, stream,
                     rec_args)
+        if self.timing is not None:  # the recovery kernel's events: from the call kernel's end to its own
+            end = torch.cuda.Event(enable_timing=True)
+            end.record()
+            self.timing_recover.append((ev[1], end))
 
     def _fp64_cells(self, B: int) -> torch.Tensor | None:
         """The mixed form's per-cell flags [B] (int32): the cells whose FP32 refinement failed in an earlier call with

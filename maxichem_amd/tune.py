@@ -1,10 +1,11 @@
 """The call kernel's autotuner: cells (wavefronts) per workgroup, the stream batch, the register budget (wavefronts
-per SIMD) and the rate coefficients' place (LDS or global scratch), per network and device.
+per SIMD) and the rate coefficients' place (LDS or global scratch), per network, device and workload.
 
-Each candidate is compiled, loaded and timed on one synthetic uneven call (solar gas from cold, dt log-uniform
-1e1-1e4 s, half of the cells lit), by HIP events around the call kernel; the fastest is stored in the code cache
-(tune.json) under the network's table digest, the photolysis switch and the device name, and later solvers of that
-network on that device take it.
+Each candidate is compiled, loaded and timed by HIP events around the call kernel on a sample of the workload:
+"uneven", one synthetic call from cold (solar gas, dt log-uniform 1e1-1e4 s, half of the cells lit), or "settled",
+calls of dt = 300 s after a dark relaxation (the GPU FP64 case's sequence, a quarter of the cells lit). The fastest
+is stored in the code cache (tune.json) under the network's table digest, the photolysis switch, the device name and
+the workload, and later solvers of that network on that device take the uneven choice unless told the workload.
 """
 
 from __future__ import annotations
@@ -27,41 +28,57 @@ def _store() -> Path:
     return STORE if STORE is not None else _hip.CACHE / "tune.json"
 
 
-def _key(net, photo: bool, device: torch.device) -> str:
-    return f"{_tables.build(net, photo).digest()}|{int(photo)}|{torch.cuda.get_device_name(device)}"
+WORKLOADS = ("uneven", "settled")
 
 
-def stored(net, device, photo: bool) -> dict | None:
+def _key(net, photo: bool, device: torch.device, workload: str = "uneven") -> str:
+    key = f"{_tables.build(net, photo).digest()}|{int(photo)}|{torch.cuda.get_device_name(device)}"
+    return key if workload == "uneven" else f"{key}|{workload}"
+
+
+def stored(net, device, photo: bool, workload: str = "uneven") -> dict | None:
     try:
-        return json.loads(_store().read_text()).get(_key(net, photo, device))
+        return json.loads(_store().read_text()).get(_key(net, photo, device, workload))
     except (OSError, ValueError):
         return None
 
 
-def choose(net, device, photo: bool, drift: float, run: bool = False) -> tuple[int, int]:
-    """The variant for this network on this device: tuned now when run, else the stored choice, else DEFAULT."""
+def choose(net, device, photo: bool, drift: float, run: bool = False, workload: str = "uneven") -> tuple[int, int]:
+    """The variant for this network, device and workload: tuned now when run, else the stored choice, else DEFAULT."""
     if run:
-        return tuple(tune(net, device, photo, drift)["best"])
-    s = stored(net, device, photo)
+        return tuple(tune(net, device, photo, drift, workload=workload)["best"])
+    s = stored(net, device, photo, workload)
     return tuple(s["best"]) if s else DEFAULT
 
 
-def tune(net, device, photo: bool, drift: float, cells: int = 8192, seed: int = 7, repeats: int = 2) -> dict:
-    """Time every variant on one uneven call; store and return {"best", "ms": {variant: ms}}."""
+def tune(net, device, photo: bool, drift: float, cells: int = 8192, seed: int = 7, repeats: int = 2,
+         workload: str = "uneven") -> dict:  # fmt: skip
+    """Time every variant on the workload's sample; store and return {"best", "ms": {variant: ms}, "private_bytes":
+    {variant: bytes per lane the variant spills}}."""
     from .kincuda import CudaSolver
 
+    if workload not in WORKLOADS:
+        raise ValueError(f"workload: one of {WORKLOADS} expected, got {workload!r}")
     dev = torch.device(device)
     nj = len(net.photo) if photo else 0
-    d = workloads.config3(net.species, len(net.photo), cells=cells, seed=seed)
-    rng = np.random.default_rng(seed)
-    dt = 10.0 ** rng.uniform(1.0, 4.0, cells)
 
     def g(a):
         return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
 
-    x, T, P, h, dtt = g(d["x"]), g(d["T"]), g(d["P"]), g(d["h"]), g(dt)
-    J = g(d["J"][:, :nj]) if photo else None
-    ms = {}
+    if workload == "uneven":
+        d = workloads.config3(net.species, len(net.photo), cells=cells, seed=seed)
+        rng = np.random.default_rng(seed)
+        dt = 10.0 ** rng.uniform(1.0, 4.0, cells)
+        x, T, P, h, dtt = g(d["x"]), g(d["T"]), g(d["P"]), g(d["h"]), g(dt)
+        J = g(d["J"][:, :nj]) if photo else None
+    else:
+        d = workloads.config1(net.species, len(net.photo), cells=cells, seed=seed)
+        T, P = g(d["T"]), g(d["P"])
+        J = g(d["J"][:, :nj]) if photo else None
+        with CudaSolver(net, dev, photo=photo, drift=drift, variant=DEFAULT) as s:
+            r = s(g(d["x"]), T, P, g(d["dt_relax"]), g(d["h"]), J=None if J is None else torch.zeros_like(J))
+            x, h, dtt = r.x, r.h, g(d["dt"])
+    ms, spills = {}, {}
     for v in VARIANTS:
         with CudaSolver(net, dev, photo=photo, drift=drift, variant=v) as s:
             s(x, T, P, dtt, h, J=J)  # the first call pays for loading
@@ -69,16 +86,19 @@ def tune(net, device, photo: bool, drift: float, cells: int = 8192, seed: int = 
             for _ in range(repeats):
                 s(x, T, P, dtt, h, J=J)
             torch.cuda.synchronize(dev)
-            ms[",".join(map(str, v))] = float(np.median([a.elapsed_time(b) for a, b in s.timing]))
+            key = ",".join(map(str, v))
+            ms[key] = float(np.median([a.elapsed_time(b) for a, b in s.timing]))
+            spills[key] = int(s.info["private_bytes"])
             s.timing = None
     best = min(ms, key=ms.get)
-    out = {"best": [int(q) for q in best.split(",")], "ms": ms, "cells": cells}
+    out = {"best": [int(q) for q in best.split(",")], "ms": ms, "private_bytes": spills, "cells": cells,
+           "workload": workload}  # fmt: skip
     path = _store()
     try:
         data = json.loads(path.read_text()) if path.exists() else {}
     except ValueError:
         data = {}
-    data[_key(net, photo, dev)] = out
+    data[_key(net, photo, dev, workload)] = out
     try:
         path.parent.mkdir(parents=True, exist_ok=True)
         tmp = path.with_suffix(f".tmp{os.getpid()}")

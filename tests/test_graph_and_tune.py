@@ -87,3 +87,11 @@ def test_the_tuned_choice_is_stored_and_taken(ncho, tmp_path, monkeypatch):
     assert tuple(tune.stored(ncho, torch.device("cuda"), True)["best"]) == tuple(out["best"])
     with CudaSolver(ncho, "cuda", photo=True) as solver:
         assert solver.info["variant"] == tuple(out["best"])
+    # the settled workload's choice is stored apart, and taken when that workload is named
+    settled = tune.tune(ncho, torch.device("cuda"), True, 1e-6, cells=2048, workload="settled")
+    print(f"autotuner, settled calls: {settled}")
+    assert settled["workload"] == "settled" and set(settled["private_bytes"]) == set(settled["ms"])
+    assert tune.choose(ncho, torch.device("cuda"), True, 1e-6, workload="settled") == tuple(settled["best"])
+    assert tune.choose(ncho, torch.device("cuda"), True, 1e-6) == tuple(out["best"])
+    with pytest.raises(ValueError, match="workload"):
+        tune.tune(ncho, torch.device("cuda"), True, 1e-6, cells=64, workload="busy")

@@ -34,6 +34,7 @@ FP64_PEAK = 78.6  # TFLOP/s, MI355X FP64 vector
 # from the FETCH_SIZE and WRITE_SIZE counters of separate profiled runs of this bench (timed calls), scaled by what
 # they read for a 512 MiB copy (x2 and x1), profiles/mxc_call_ncho_65536_traffic.md; not measured in this run
 TRAFFIC_PER_CELL_CALL = {"NCHO": 2054}
+HEADLINE_MARGIN = 1.02  # another form heads the line only when it beats the plain one by more than this
 TRAFFIC_SOURCE = "profiles/mxc_call_ncho_65536_traffic.md (rocprofv3 FETCH_SIZE, WRITE_SIZE; a constant, not this run)"
 
 
@@ -595,8 +596,14 @@ def main() -> int:
                                               and agree[n]["max_rel_species_above_1e-20"] <= 1e-2)}  # fmt: skip
     if not ok:  # no plain form timed (--form learned, ...): nothing to check against; the fastest form measured
         ok = set(runs)
-    headline = min(ok, key=lambda n: runs[n]["wall"] / len(runs[n]["tries"]))
+    per_call = {n: runs[n]["wall"] / len(runs[n]["tries"]) for n in runs}
+    headline = min(ok, key=per_call.get)
+    if "plain" in runs and per_call[headline] > per_call["plain"] / HEADLINE_MARGIN:
+        headline = "plain"  # a form within the run-to-run spread of the plain one does not replace it
     main_run = runs[headline]
+    # the constant measures the FP64 call kernel, which the plain and learned forms share (the learned form changes
+    # only the starting h); the mixed forms run another kernel
+    fp64_call = headline in ("plain", "learned") and args.network in TRAFFIC_PER_CELL_CALL
     line = {
         "metric": "cells/s per call (NCHO, 64K cells): learned+FP32-LU vs plain FP64 at same error",
         "value": args.cells * len(main_run["tries"]) / main_run["wall"],
@@ -624,9 +631,8 @@ def main() -> int:
                        "kernel issues no MFMA)",
             "achieved": main_run["achieved"], "peak": FP64_PEAK, "unit": "TFLOP/s",
             "frac": main_run["achieved"] / FP64_PEAK,
-            "traffic": (TRAFFIC_PER_CELL_CALL[args.network] * args.cells
-                        if headline == "plain" and args.network in TRAFFIC_PER_CELL_CALL else None),
-            "traffic_source": TRAFFIC_SOURCE if headline == "plain" and args.network in TRAFFIC_PER_CELL_CALL else None,
+            "traffic": TRAFFIC_PER_CELL_CALL[args.network] * args.cells if fp64_call else None,
+            "traffic_source": TRAFFIC_SOURCE if fp64_call else None,
             "kernel": "mxc_call", "kernel_ms": float(main_run["kernel_ms"].mean()),
             "flop_per_try": work.per_try, "flop_rates_per_cell": work.rates,
             "tries_per_launch": float(np.mean([t.sum() for t in main_run["tries"]])),

@@ -18,9 +18,11 @@ from ._tables import (
     SV_END,
     SV_NEG,
     SV_PIECE,
+    SV_PIECE_BITS,
     SV_R_BITS,
     T_DBL,
     T_DEST,
+    T_DEST_BITS,
     T_END,
     T_IDX_BITS,
     T_NEG,
@@ -110,6 +112,8 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixe
     lines += [
         f"constexpr unsigned T_IDX_MASK = {(1 << T_IDX_BITS) - 1}u;",
         f"constexpr unsigned SV_R_MASK = {(1 << SV_R_BITS) - 1}u;",
+        f"constexpr unsigned T_DEST_MASK = {(1 << T_DEST_BITS) - 1}u;",
+        f"constexpr unsigned SV_PIECE_MASK = {(1 << SV_PIECE_BITS) - 1}u;",
         f"__device__ constexpr int CH_ROUNDS[{nch}] = {{{', '.join(map(str, rounds))}}};",
         f"__device__ constexpr int CH_OFF[{nch}] = {{{', '.join(map(str, offsets))}}};",
         # the trailing rows' filled pattern: bit j of TRAIL_FILL[t] is column K0 + j of row K0 + t

@@ -43,11 +43,14 @@ from .kinvulcan import VulcanNetwork, _check_m
 
 MAX_SPECIES = 128  # the lead part of a state vector is one register per lane: K0 = NS - 64 <= 64
 CHUNK = 8  # trailing-block columns assembled per staging pass
-# Term word of the Jacobian stream (uint32): dv index, sign, x2, end of entry, destination (LDS double offset)
-T_IDX_BITS, T_NEG, T_DBL, T_END, T_DEST = 16, 16, 17, 18, 19
+# Term word of the Jacobian stream (uint32): dv index [0, 16), end of entry 16, x2 17, destination (LDS double
+# offset) [18, 31), sign 31. The sign sits in the word's sign bit, where a kernel takes it with one AND to flip a
+# double's sign; x2 is an exponent step. A term is then exact without a compare or select.
+T_IDX_BITS, T_END, T_DBL, T_DEST, T_NEG = 16, 16, 17, 18, 31
 T_DEST_BITS = 13
-# S v stream word (uint32): reaction, sign, x2, end of piece, piece index
-SV_R_BITS, SV_NEG, SV_DBL, SV_END, SV_PIECE = 11, 11, 12, 13, 14
+# S v stream word (uint32): reaction [0, 11), end of piece 11, x2 12, piece index [13, 31), sign 31
+SV_R_BITS, SV_END, SV_DBL, SV_PIECE, SV_NEG = 11, 11, 12, 13, 31
+SV_PIECE_BITS = 18
 
 
 def _log(a: float) -> float:
@@ -351,6 +354,8 @@ def build(net: VulcanNetwork, photo: bool = False) -> Tables:
             pieces.append(t[c : c + share])
         row_pieces[i] = first | (len(pieces) - first) << 16
     tab.npart = max(1, len(pieces))
+    if tab.npart >= (1 << SV_PIECE_BITS):
+        raise ValueError(f"{tab.npart} S v pieces exceed the stream's piece field")
     tab.row_pieces = row_pieces
     tab.sv_terms = total
     lanes = _balance([len(p) for p in pieces], WAVE)

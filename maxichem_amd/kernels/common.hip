@@ -134,6 +134,16 @@ __device__ __forceinline__ double coefficient(u32 w, int neg, int dbl) {
   return ((w >> neg) & 1u) ? -c : c;
 }
 
+// A stream word's term: d times the word's coefficient, exactly as fma(coefficient, d, .) takes it: x2 is an exponent
+// step, and the sign (the word's bit 31) is XORed into the double's sign bit. Four instructions, where the
+// coefficient's compares and selects take eight and wait states for their condition masks.
+static_assert(T_NEG == 31 && SV_NEG == 31, "the streams' sign bit is the word's sign bit");
+template <int DBL>
+__device__ __forceinline__ double term(u32 w, double d) {
+  d = __builtin_amdgcn_ldexp(d, (int)((w >> DBL) & 1u));
+  return __longlong_as_double(__double_as_longlong(d) ^ ((long long)(w & 0x80000000u) << 32));
+}
+
 // The network's tables in device memory, one pointer for every kernel (a function reads the fields it needs through
 // opaque(net), so they are loaded where used rather than held in registers across the call).
 struct Net {
@@ -305,9 +315,9 @@ __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& 
     for (int i = 0; i < STREAM_BATCH; i++) d[i] = V[w[i] & SV_R_MASK];
 #pragma unroll
     for (int i = 0; i < STREAM_BATCH; i++) {
-      acc = fma(coefficient(w[i], SV_NEG, SV_DBL), d[i], acc);
+      acc += term<SV_DBL>(w[i], d[i]);
       if ((w[i] >> SV_END) & 1u) {
-        part[w[i] >> SV_PIECE] = acc;
+        part[(w[i] >> SV_PIECE) & SV_PIECE_MASK] = acc;
         acc = 0.0;
       }
     }
@@ -348,9 +358,9 @@ __device__ __forceinline__ void jac_stream(const u32* __restrict__ words, int ro
     for (int i = 0; i < STREAM_BATCH; i++) d[i] = DV[w[i] & T_IDX_MASK];
 #pragma unroll
     for (int i = 0; i < STREAM_BATCH; i++) {
-      acc = fma(coefficient(w[i], T_NEG, T_DBL), d[i], acc);
+      acc += term<T_DBL>(w[i], d[i]);
       if ((w[i] >> T_END) & 1u) {
-        ws[w[i] >> T_DEST] = acc;
+        ws[(w[i] >> T_DEST) & T_DEST_MASK] = acc;
         acc = 0.0;
       }
     }

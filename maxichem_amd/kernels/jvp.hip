@@ -202,7 +202,7 @@ __device__ __forceinline__ void rhs_tangent(double Yl, double Yt, double Yel, do
 #pragma unroll
     for (int p = 0; p < 4; p++) acc[p] = fma(c, V[p * SV + r], acc[p]);
     if ((w >> SV_END) & 1u) {
-      const int piece = (int)(w >> SV_PIECE);
+      const int piece = (int)((w >> SV_PIECE) & SV_PIECE_MASK);
 #pragma unroll
       for (int p = 0; p < 4; p++) {
         part[p * NPART + piece] = acc[p];

@@ -124,9 +124,9 @@ __device__ __forceinline__ void m0_times(const double (&xl)[NV], const double (&
       for (int i = 0; i < STREAM_BATCH; i++) dd[i] = V[wd[i] & SV_R_MASK];
 #pragma unroll
       for (int i = 0; i < STREAM_BATCH; i++) {
-        acc = fma(coefficient(wd[i], SV_NEG, SV_DBL), dd[i], acc);
+        acc += term<SV_DBL>(wd[i], dd[i]);
         if ((wd[i] >> SV_END) & 1u) {
-          part[wd[i] >> SV_PIECE] = acc;
+          part[(wd[i] >> SV_PIECE) & SV_PIECE_MASK] = acc;
           acc = 0.0;
         }
       }

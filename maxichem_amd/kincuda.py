@@ -89,7 +89,7 @@ class CudaSolver(Solver):
             if variant is None:
                 from . import tune as _tune
 
-                variant = _tune.choose(net, self.dev, photo, self.drift, run=tune)
+                variant = _tune.choose(net, self.dev, photo, self.drift, run=tune, mixed=precision == "mixed")
             self._setup(net, photo, variant)
 
     def _setup(self, net, photo: bool, variant: tuple[int, ...]) -> None:

@@ -5,7 +5,13 @@ import numpy as np
 import pytest
 
 from maxichem_amd import _tables, standin
-from maxichem_amd._tables import CHUNK, SV_END, SV_NEG, SV_DBL, SV_PIECE, T_DBL, T_DEST, T_END, T_NEG
+from maxichem_amd._tables import (CHUNK, SV_DBL, SV_END, SV_NEG, SV_PIECE, SV_PIECE_BITS, SV_R_BITS, T_DEST,
+                                  T_DEST_BITS, T_END)
+
+
+def field(w, at: int, bits: int) -> int:
+    """The bits [at, at + bits) of a stream word."""
+    return (int(w) >> at) & ((1 << bits) - 1)
 
 
 def _nets(read_network):
@@ -45,12 +51,12 @@ def test_streams_decode_to_s_v_and_minus_jn(read_network):
         for lane in range(64):
             acc = {}
             for w in tab.sv_words[:, lane]:
-                r = int(w) & ((1 << 11) - 1)
+                r = field(w, 0, SV_R_BITS)
                 c = (2 if (w >> SV_DBL) & 1 else 1) * (-1 if (w >> SV_NEG) & 1 else 1)
                 if r < tab.nr:
                     acc[r] = acc.get(r, 0) + c
                 if (w >> SV_END) & 1:
-                    pieces[int(w) >> SV_PIECE] = acc
+                    pieces[field(w, SV_PIECE, SV_PIECE_BITS)] = acc
                     acc = {}
         for i in range(ns):
             first, count = int(tab.row_pieces[i]) & 0xFFFF, int(tab.row_pieces[i]) >> 16
@@ -69,7 +75,7 @@ def test_streams_decode_to_s_v_and_minus_jn(read_network):
                     if (int(w) & 0xFFFF) != tab.ndv:
                         terms += 1
                     if (w >> T_END) & 1:
-                        dests.append((id(words), int(w) >> T_DEST))
+                        dests.append((id(words), field(w, T_DEST, T_DEST_BITS)))
         assert len(dests) == len(set(dests))
         assert terms == tab.jac_terms
 
@@ -105,7 +111,7 @@ def test_chunks_stage_the_trailing_rows_by_column(read_network):
         for c, words in enumerate(tab.chunk_words):
             for w in words.ravel():
                 if (w >> T_END) & 1:
-                    off = (int(w) >> T_DEST) - L["STAGE"]
+                    off = field(w, T_DEST, T_DEST_BITS) - L["STAGE"]
                     assert 0 <= off < CHUNK * tab.nt
                     slot, t = divmod(off, tab.nt)
                     col = c * CHUNK + slot if c < tab.nlch else tab.k0 + (c - tab.nlch) * CHUNK + slot

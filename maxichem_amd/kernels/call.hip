@@ -16,7 +16,7 @@ __device__ __forceinline__ void factor_m0(double (&c)[NT], LeadRow& a21, double&
 template <int NV>
 __device__ __forceinline__ bool solve_m0(double (&yl)[NV], double (&yt)[NV], const double (&c)[NT],
                                          const LeadRow& a21, double dl, double dtr, double, double, double,
-                                         const Net*, double* ws, int lane) {
+                                         NetP, double* ws, int lane) {
   solve<NV>(yl, yt, c, a21, dl, dtr, ws, lane);
   return true;
 }
@@ -29,7 +29,7 @@ __device__ __forceinline__ void factor_m0(float (&c)[NT], LeadRow& a21, double& 
 
 template <int NV>
 __device__ __forceinline__ bool solve_m0(double (&yl)[NV], double (&yt)[NV], const float (&c)[NT], const LeadRow& a21,
-                                         double dl, float dtr, double igh, double wl, double wt, const Net* net,
+                                         double dl, float dtr, double igh, double wl, double wt, NetP net,
                                          double* ws, int lane) {
   return msolve<NV>(yl, yt, c, a21, dl, dtr, igh, wl, wt, net, ws, lane);
 }
@@ -46,7 +46,7 @@ struct StageKeep {
 // keep, when not null, receives StageKeep's values (the arithmetic is the same either way).
 template <typename T, typename DT = T>
 __device__ __forceinline__ bool stages(T (&c)[NT], LeadRow& a21, double Nl, double Nt, double ha, double rtol,
-                                       double atol, const Net* net, double nd, double* ws, const double* rk,
+                                       double atol, NetP net, double nd, double* ws, const double* rk,
                                        int lane, double& K1l, double& K1t, double& K3l, double& K3t, double& K4l,
                                        double& K4t, StageKeep* keep = nullptr, DT* dtr_keep = nullptr) {
   const double igh = 1.0 / (GAMMA * ha), ih = 1.0 / ha;
@@ -97,7 +97,7 @@ __device__ __forceinline__ bool stages(T (&c)[NT], LeadRow& a21, double Nl, doub
 }
 
 extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_waves_per_eu(CALL_WAVES_PER_SIMD)))
-    mxc_call(const Net* __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
+    mxc_call(NetP __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
              const double* __restrict__ Pa, const double* __restrict__ dta, double* __restrict__ hio,
              double* __restrict__ xout, int* __restrict__ doneo, double* __restrict__ noko,
              int* __restrict__ status, int* __restrict__ redo, int* __restrict__ list, int* __restrict__ count,
@@ -120,7 +120,7 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_wa
     const int cell = order ? order[q] : q;
     const double Tc = Ta[cell], dt = dta[cell], nd = Pa[cell] / (KBOLTZ * Tc);
     const double* xr = x0 + (size_t)cell * NS;
-    const int* perm = opaque(net)->perm;
+    const gptr<int> perm = opaque(net)->perm;
     double Nl = lane < K0 ? xr[perm[lane]] : 0.0, Nt = lane < NT ? xr[perm[K0 + lane]] : 0.0;
     // the screen (common.cuh:244-279): a NaN enters the sum for a negative or non-finite x or J
     double p = (lane < K0 ? (Nl >= 0.0 ? Nl : DNAN) : 0.0) + (lane < NT ? (Nt >= 0.0 ? Nt : DNAN) : 0.0);

@@ -144,26 +144,35 @@ __device__ __forceinline__ double term(u32 w, double d) {
   return __longlong_as_double(__double_as_longlong(d) ^ ((long long)(w & 0x80000000u) << 32));
 }
 
+// A table pointer in the global address space. Loads through it are global loads, counted by the vector-memory
+// counter alone, which the compiler may issue ahead of LDS accesses and wait for only where their values are used.
+// Through a plain (flat) pointer, which may address LDS, each load was ordered with the wave's LDS traffic and waited
+// for at once: a round of reaction words cost a memory latency.
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T*;
+
 // The network's tables in device memory, one pointer for every kernel (a function reads the fields it needs through
 // opaque(net), so they are loaded where used rather than held in registers across the call).
 struct Net {
-  const int* perm;      // [NS] network species of elimination row k
-  const double* rc;     // [NR][RCW] rate-law rows
-  const int* rf;        // [NR] law flags
-  const double* nasa;   // [2][NS][9] NASA-9 ranges, elimination order
-  const u64* rxe;       // [NR] six species (elimination order, pad NS) and the reaction's first dv slot
-  const u32* svw;       // S v stream [SV_ROUNDS][WAVE]
-  const u32* rowp;      // [NS] each row's first S v piece | count << 16
-  const u32* diagw;     // -Jn streams: diagonal, lead, trailing chunks
-  const u32* leadw;
-  const u32* chw;
-  const double* atoms;  // [NS][NEL] network order
-  const int* rord;      // [NR] the order the rate coefficients are computed in (reactions grouped by law)
+  gptr<int> perm;      // [NS] network species of elimination row k
+  gptr<double> rc;     // [NR][RCW] rate-law rows
+  gptr<int> rf;        // [NR] law flags
+  gptr<double> nasa;   // [2][NS][9] NASA-9 ranges, elimination order
+  gptr<u64> rxe;       // [NR] six species (elimination order, pad NS) and the reaction's first dv slot
+  gptr<u32> svw;       // S v stream [SV_ROUNDS][WAVE]
+  gptr<u32> rowp;      // [NS] each row's first S v piece | count << 16
+  gptr<u32> diagw;     // -Jn streams: diagonal, lead, trailing chunks
+  gptr<u32> leadw;
+  gptr<u32> chw;
+  gptr<double> atoms;  // [NS][NEL] network order
+  gptr<int> rord;      // [NR] the order the rate coefficients are computed in (reactions grouped by law)
 };
+// The Net struct in the constant address space: its fields come by scalar loads, uniform across the wave.
+using NetP = const __attribute__((address_space(4))) Net*;
 
 // ---- rate coefficients (the reference's law, collision_limit, gibbs_rt and rate_pair: common.cuh:135-242) ----
 
-__device__ __forceinline__ double law(const double* __restrict__ q, int fl, double T, double lT, double iT, double nd) {
+__device__ __forceinline__ double law(gptr<double> __restrict__ q, int fl, double T, double lT, double iT, double nd) {
   double k = exp(q[0] + q[1] * lT - q[2] * iT);
   if (fl & FL_SECOND_K0) k += exp(q[3] + q[4] * lT - q[5] * iT);
   if (fl & FL_FALLOFF) {
@@ -192,7 +201,7 @@ __device__ __forceinline__ double bimolecular(int m, bool three_body, double k, 
 }
 
 // G / RT of one species from its NASA-9 range a at T.
-__device__ __forceinline__ double gibbs_rt(const double* __restrict__ a, double T, double lT, double iT) {
+__device__ __forceinline__ double gibbs_rt(gptr<double> __restrict__ a, double T, double lT, double iT) {
   const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
   const double h = -a[0] * iT * iT + a[1] * lT * iT + a[2] + a[3] * T / 2 + a[4] * T2 / 3 + a[5] * T3 / 4 +
                    a[6] * T4 / 5 + a[7] * iT;
@@ -206,21 +215,21 @@ __device__ __forceinline__ int species_of(u64 w, int m) { return (int)((w >> (8 
 // The rate coefficients of every reaction at the cell's T and nd into rk (kf [NR] then kr [NR]; LDS, or the wave's
 // slot of global scratch), kept there for the call.
 // G / RT (elimination order) goes to the scratch at L_UNION first.
-__device__ __forceinline__ void cell_rates(double* ws, double* rk, double T, double nd, const Net* net,
+__device__ __forceinline__ void cell_rates(double* ws, double* rk, double T, double nd, NetP net,
                                            const double* Jc, int lane) {
   net = opaque(net);
-  const double* rc = net->rc;
-  const int* rf = net->rf;
-  const u64* rxe = net->rxe;
+  const gptr<double> rc = net->rc;
+  const gptr<int> rf = net->rf;
+  const gptr<u64> rxe = net->rxe;
   double* g = ws + L_UNION;
   double* kf = rk;
   double* kr = rk + NR;
   const double lT = log(T), iT = 1.0 / T, lKT = log(KBOLTZ * T / P0BAR);
-  const double* cf = net->nasa + (T < NASA_SPLIT_T ? 0 : NS * 9);
+  const gptr<double> cf = net->nasa + (T < NASA_SPLIT_T ? 0 : NS * 9);
   for (int i = lane; i < NS; i += WAVE) g[i] = gibbs_rt(cf + 9 * i, T, lT, iT);
   if (lane == 0) g[NS] = 0.0;
   wave_sync();
-  const int* rord = net->rord;
+  const gptr<int> rord = net->rord;
 #pragma unroll 1
   for (int q = lane; q < NR; q += WAVE) {
     const int r = rord[q];  // a round's reactions mostly of one law, so it runs the code of few laws
@@ -253,11 +262,44 @@ __device__ __forceinline__ void cell_rates(double* ws, double* rk, double T, dou
 
 // ---- right-hand side and Jacobian ----
 
+// A term stream (the Jacobian's and S v's): each lane sums whole entries from src terms and writes each at its
+// destination in dst. The words of STREAM_BATCH rounds are loaded together, ahead of their terms, so that their
+// latency is paid once per batch; rounds past the end take the padding word pad (a zero term, no end). (Loading each
+// batch, or a stream's first batch, one step ahead was measured slower: it holds 36 more registers.)
+template <unsigned IDX_MASK, int DBL, int END, int DEST, unsigned DEST_MASK>
+__device__ __forceinline__ void stream_sum(gptr<u32> __restrict__ words, int rounds, u32 pad, const double* src,
+                                           double* dst, int lane) {
+  double acc = 0.0;
+#pragma unroll 1
+  for (int q0 = 0; q0 < rounds; q0 += STREAM_BATCH) {
+    u32 w[STREAM_BATCH];
+    double d[STREAM_BATCH];
+#pragma unroll
+    for (int i = 0; i < STREAM_BATCH; i++) w[i] = q0 + i < rounds ? words[(q0 + i) * WAVE + lane] : pad;
+#pragma unroll
+    for (int i = 0; i < STREAM_BATCH; i++) d[i] = src[w[i] & IDX_MASK];
+#pragma unroll
+    for (int i = 0; i < STREAM_BATCH; i++) {
+      acc += term<DBL>(w[i], d[i]);
+      if ((w[i] >> END) & 1u) {
+        dst[(w[i] >> DEST) & DEST_MASK] = acc;
+        acc = 0.0;
+      }
+    }
+  }
+}
+
+// One Jacobian stream into the cell's LDS: entries of -Jn at their offsets.
+__device__ __forceinline__ void jac_stream(gptr<u32> __restrict__ words, int rounds, const double* DV, double* ws,
+                                           int lane) {
+  stream_sum<T_IDX_MASK, T_DBL, T_END, T_DEST, T_DEST_MASK>(words, rounds, (u32)NDV, DV, ws, lane);
+}
+
 // F = (n_tot / nd) S v(n) at the state Y (lead / trailing parts), u0 = S v / nd; with JAC also the partial
 // derivatives dv/dn of every live reaction into DV. Returns n_tot = sum Y.
 template <bool JAC>
 __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& Ft, double& u0l, double& u0t,
-                                      const Net* net, double nd, double* ws, const double* rk, int lane) {
+                                      NetP net, double nd, double* ws, const double* rk, int lane) {
   const double ntot = wave_sum((lane < K0 ? Yl : 0.0) + (lane < NT ? Yt : 0.0));
   const double xf = nd / ntot, ixf = ntot / nd;
   double* n = ws + L_NVEC;
@@ -266,26 +308,36 @@ __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& 
   if (lane == 0) n[NS] = 1.0;  // the padding species of a shorter side
   wave_sync();
   net = opaque(net);
-  const u64* rxe = net->rxe;
-  const int* rf = net->rf;
-  const u32* svw = net->svw;
-  const u32* rowp = net->rowp;
+  const gptr<u64> rxe = net->rxe;
+  const gptr<int> rf = net->rf;
+  const gptr<u32> svw = net->svw;
+  const gptr<u32> rowp = net->rowp;
   const double* kf = rk;
   const double* kr = rk + NR;
   double* V = ws + L_V;
   double* DV = ws + L_DV;
-  // unrolled, so that the reaction words of every round are loaded ahead of their use
+  // every round's reaction word (and law flags) loaded first: the loads are in bounds for every lane (the last round's
+  // spare lanes read reaction NR - 1), so none waits on the branch that ends a lane's rounds
+  u64 wr[RROUNDS];
+  int fr[JAC ? RROUNDS : 1];
+  const u32 rpl = rowp[lane < K0 ? lane : 0], rpt = rowp[K0 + (lane < NT ? lane : 0)];  // each row's S v pieces
+#pragma unroll
+  for (int q = 0; q < RROUNDS; q++) {
+    const int r = q * WAVE + lane < NR ? q * WAVE + lane : NR - 1;
+    wr[q] = rxe[r];
+    if (JAC) fr[q] = rf[r];
+  }
 #pragma unroll
   for (int q = 0; q < RROUNDS; q++) {
     const int r = q * WAVE + lane;
     if (r >= NR) break;
-    const u64 w = rxe[r];
+    const u64 w = wr[q];
     const double n0 = n[species_of(w, 0)], n1 = n[species_of(w, 1)], n2 = n[species_of(w, 2)];
     const double n3 = n[species_of(w, 3)], n4 = n[species_of(w, 4)], n5 = n[species_of(w, 5)];
     const double f = kf[r], b = kr[r];
     V[r] = f * n0 * n1 * n2 - b * n3 * n4 * n5;
     if (JAC) {
-      const int fl = rf[r];
+      const int fl = fr[JAC ? q : 0];
       if (!(fl & FL_INERT)) {
         int o = (int)(w >> 48);
         const int mf = (fl >> FL_NF_SHIFT) & 3, mr = (fl >> FL_NR_SHIFT) & 3;
@@ -304,34 +356,16 @@ __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& 
   if (lane == 0) V[NR] = 0.0;  // the S v stream's padding term
   wave_sync();
   double* part = ws + L_PART;
-  double acc = 0.0;
-#pragma unroll 1
-  for (int q0 = 0; q0 < SV_ROUNDS; q0 += STREAM_BATCH) {  // as jac_stream: a batch of rounds' words at a time
-    u32 w[STREAM_BATCH];
-    double d[STREAM_BATCH];
-#pragma unroll
-    for (int i = 0; i < STREAM_BATCH; i++) w[i] = q0 + i < SV_ROUNDS ? svw[(q0 + i) * WAVE + lane] : (u32)NR;
-#pragma unroll
-    for (int i = 0; i < STREAM_BATCH; i++) d[i] = V[w[i] & SV_R_MASK];
-#pragma unroll
-    for (int i = 0; i < STREAM_BATCH; i++) {
-      acc += term<SV_DBL>(w[i], d[i]);
-      if ((w[i] >> SV_END) & 1u) {
-        part[(w[i] >> SV_PIECE) & SV_PIECE_MASK] = acc;
-        acc = 0.0;
-      }
-    }
-  }
+  stream_sum<SV_R_MASK, SV_DBL, SV_END, SV_PIECE, SV_PIECE_MASK>(svw, SV_ROUNDS, (u32)NR, V, part, lane);
   wave_sync();
   if (lane < K0) {
-    const u32 rp = rowp[lane];
     double s = 0.0;
-    for (int e = 0; e < (int)(rp >> 16); e++) s += part[(rp & 0xffffu) + e];
+    for (int e = 0; e < (int)(rpl >> 16); e++) s += part[(rpl & 0xffffu) + e];
     Fl = s * ixf;
     u0l = s / nd;
   }
   if (lane < NT) {
-    const u32 rp = rowp[K0 + lane];
+    const u32 rp = rpt;
     double s = 0.0;
     for (int e = 0; e < (int)(rp >> 16); e++) s += part[(rp & 0xffffu) + e];
     Ft = s * ixf;
@@ -341,38 +375,12 @@ __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& 
   return ntot;
 }
 
-// One Jacobian stream: each lane sums whole entries of -Jn from dv terms and writes each at its LDS offset. The
-// words of STREAM_BATCH rounds are loaded together, ahead of their terms, so that their latency is paid once per
-// batch; rounds past the end take the padding word (the zero term, no end).
-
-__device__ __forceinline__ void jac_stream(const u32* __restrict__ words, int rounds, const double* DV, double* ws,
-                                           int lane) {
-  double acc = 0.0;
-#pragma unroll 1
-  for (int q0 = 0; q0 < rounds; q0 += STREAM_BATCH) {
-    u32 w[STREAM_BATCH];
-    double d[STREAM_BATCH];
-#pragma unroll
-    for (int i = 0; i < STREAM_BATCH; i++) w[i] = q0 + i < rounds ? words[(q0 + i) * WAVE + lane] : (u32)NDV;
-#pragma unroll
-    for (int i = 0; i < STREAM_BATCH; i++) d[i] = DV[w[i] & T_IDX_MASK];
-#pragma unroll
-    for (int i = 0; i < STREAM_BATCH; i++) {
-      acc += term<T_DBL>(w[i], d[i]);
-      if ((w[i] >> T_END) & 1u) {
-        ws[(w[i] >> T_DEST) & T_DEST_MASK] = acc;
-        acc = 0.0;
-      }
-    }
-  }
-}
-
 // -Jn into LEAD (LDS), the trailing rows' lead columns a21 and the trailing block c (registers), from the dv terms
 // that rhs<true> left in DV; then u = w = S v / nd - Jn N / n_tot, the constant-pressure part of the exact Jacobian
 // (lead / trailing parts).
 template <typename T>
 __device__ __forceinline__ void jacobian(T (&c)[NT], LeadRow& a21, double Nl, double Nt, double u0l, double u0t,
-                                         double ntot, double& ul, double& ut, const Net* net, double* ws, int lane) {
+                                         double ntot, double& ul, double& ut, NetP net, double* ws, int lane) {
   double* DV = ws + L_DV;
   for (int s = lane; s < L_NVEC; s += WAVE) ws[s] = 0.0;  // LEAD
   if (lane < NT) ws[L_DIAG + lane] = 0.0;                   // a species in no reaction has no diagonal term
@@ -739,9 +747,9 @@ __device__ __forceinline__ double try_error(double Nl, double Nt, double K1l, do
 }
 
 // x = N / sum N at the cell's output row (network order); returns the sum.
-__device__ __forceinline__ void write_state(double Nl, double Nt, double* __restrict__ out, const Net* net,
+__device__ __forceinline__ void write_state(double Nl, double Nt, double* __restrict__ out, NetP net,
                                             int lane) {
-  const int* perm = opaque(net)->perm;
+  const gptr<int> perm = opaque(net)->perm;
   const double s = vsum(Nl, Nt, lane);
   if (lane < K0) out[perm[lane]] = Nl / s;
   if (lane < NT) out[perm[K0 + lane]] = Nt / s;

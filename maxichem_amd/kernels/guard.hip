@@ -27,10 +27,10 @@ constexpr int NSYM = NEL * (NEL + 1) / 2;
 // positive makes the cell non-finite (x g is written as it is).
 __device__ __forceinline__ int guard_wave(double Nl, double Nt, const double* __restrict__ x0,
                                           double* __restrict__ out, bool done, double nok, double atol,
-                                          const Net* net, int lane) {
+                                          NetP net, int lane) {
   net = opaque(net);
-  const int* perm = net->perm;
-  const double* atoms = net->atoms;
+  const gptr<int> perm = net->perm;
+  const gptr<double> atoms = net->atoms;
   const bool hl = lane < K0, ht = lane < NT;
   const int il = hl ? perm[lane] : 0, it = ht ? perm[K0 + lane] : 0;
   // the input's entries and each species' atoms, loaded before the state's sum is waited for

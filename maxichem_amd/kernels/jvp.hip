@@ -74,18 +74,18 @@ __device__ __forceinline__ Dual law_dual(const double* __restrict__ q, int fl, D
 // The tangents of every reaction's coefficients (kfdot, krdot into rkd [2 NR]) from the tangents of T, nd and J (Jd).
 // The coefficients follow the call's cell_rates: G / RT from NASA-9, whose T-derivative is -H / (R T^2); the
 // collision limit's factor takes the tangent of the larger direction (half of each at a tie).
-__device__ __forceinline__ void rates_tangent(double* ts, double* rkd, double T, double Td, double nd, double ndd, const Net* net,
+__device__ __forceinline__ void rates_tangent(double* ts, double* rkd, double T, double Td, double nd, double ndd, NetP net,
                               const double* Jdc, int lane) {
   net = opaque(net);
-  const double* rc = net->rc;
-  const int* rf = net->rf;
-  const u64* rxe = net->rxe;
+  const gptr<double> rc = net->rc;
+  const gptr<int> rf = net->rf;
+  const gptr<u64> rxe = net->rxe;
   double* g = ts + J_G;    // G / RT
   double* gd = g + NS + 1;  // its tangent
   const double lT = log(T), iT = 1.0 / T, lKT = log(KBOLTZ * T / P0BAR);
-  const double* cf = net->nasa + (T < NASA_SPLIT_T ? 0 : NS * 9);
+  const gptr<double> cf = net->nasa + (T < NASA_SPLIT_T ? 0 : NS * 9);
   for (int i = lane; i < NS; i += WAVE) {
-    const double* a = cf + 9 * i;
+    const gptr<double> a = cf + 9 * i;
     g[i] = gibbs_rt(a, T, lT, iT);
     const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
     const double h = -a[0] * iT * iT + a[1] * lT * iT + a[2] + a[3] * T / 2 + a[4] * T2 / 3 + a[5] * T3 / 4 +
@@ -145,7 +145,7 @@ __device__ __forceinline__ Hyper hmul(Hyper a, Hyper b) {
 // each) and the call's rate coefficients rk and their tangents rkd.
 template <bool HD>
 __device__ __forceinline__ void rhs_tangent(double Yl, double Yt, double Yel, double Yet, double Kl, double Kt, double& Fel,
-                            double& Fet, double& Fedl, double& Fedt, const Net* net, double nd, double ndd,
+                            double& Fet, double& Fedl, double& Fedt, NetP net, double nd, double ndd,
                             double* ts, const double* rk, const double* rkd, int lane) {
   const bool hl = lane < K0, ht = lane < NT;
   double sums[3] = {(hl ? Yl : 0.0) + (ht ? Yt : 0.0), (hl ? Yel : 0.0) + (ht ? Yet : 0.0),
@@ -173,9 +173,9 @@ __device__ __forceinline__ void rhs_tangent(double Yl, double Yt, double Yel, do
   }
   wave_sync();
   net = opaque(net);
-  const u64* rxe = net->rxe;
-  const u32* svw = net->svw;
-  const u32* rowp = net->rowp;
+  const gptr<u64> rxe = net->rxe;
+  const gptr<u32> svw = net->svw;
+  const gptr<u32> rowp = net->rowp;
   double* V = ts + J_V;  // [4][NR + 1]
   constexpr int SV = NR + 1;
   auto nh = [&](int i) { return Hyper{n[i], n[SN + i], HD ? n[2 * SN + i] : 0.0, HD ? n[3 * SN + i] : 0.0}; };
@@ -245,10 +245,10 @@ __device__ __forceinline__ void w_solve(double& yl, double& yt, const double (&c
 // input x0 and its tangent x0d, for a cell that passed the guard: x = N / sum N, then x' = x g / sum x g with
 // g = clamp(1 + sum_e lambda_e a_e) and (A^T X A) lambda = e0 s1 / s0 - e1 (live elements as the guard takes them).
 __device__ __forceinline__ void guard_tangent(double Nl, double Nt, double Ndl, double Ndt, const double* __restrict__ x0,
-                              const double* __restrict__ x0d, double* __restrict__ out, const Net* net, int lane) {
+                              const double* __restrict__ x0d, double* __restrict__ out, NetP net, int lane) {
   net = opaque(net);
-  const int* perm = net->perm;
-  const double* atoms = net->atoms;
+  const gptr<int> perm = net->perm;
+  const gptr<double> atoms = net->atoms;
   const bool hl = lane < K0, ht = lane < NT;
   const int il = hl ? perm[lane] : 0, it = ht ? perm[K0 + lane] : 0;
   double Al[NEL], At[NEL];
@@ -370,7 +370,7 @@ __device__ __forceinline__ void guard_tangent(double Nl, double Nt, double Ndl, 
 // Output: xd [B][NS], the tangent of the guard's result (network order); a cell the guard fails gets NaN there (the
 // recovery recomputes it without a tangent).
 extern "C" __global__ void __launch_bounds__(WAVE* CPW)
-    mxc_jvp(const Net* __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
+    mxc_jvp(NetP __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
             const double* __restrict__ Pa, const double* __restrict__ dta, double* __restrict__ hio,
             double* __restrict__ xout, int* __restrict__ doneo, double* __restrict__ noko, int* __restrict__ status,
             int* __restrict__ redo, int* __restrict__ list, int* __restrict__ count, double* __restrict__ tries,
@@ -392,7 +392,7 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
     const double Tc = Ta[cell], dt = dta[cell], nd = Pa[cell] / (KBOLTZ * Tc);
     const double* xr = x0 + (size_t)cell * NS;
     const double* xdr = x0d + (size_t)cell * NS;
-    const int* perm = opaque(net)->perm;
+    const gptr<int> perm = opaque(net)->perm;
     double Nl = lane < K0 ? xr[perm[lane]] : 0.0, Nt = lane < NT ? xr[perm[K0 + lane]] : 0.0;
     double Ndl = lane < K0 ? xdr[perm[lane]] : 0.0, Ndt = lane < NT ? xdr[perm[K0 + lane]] : 0.0;
     // the call kernel's screen

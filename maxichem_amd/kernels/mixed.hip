@@ -68,7 +68,7 @@ __device__ __forceinline__ void solve_trailing32(double (&ytd)[NV], const float 
 // terms of the try's start (DV) and the S v stream; the exchange vectors XV hold x by species.
 template <int NV>
 __device__ __forceinline__ void m0_times(const double (&xl)[NV], const double (&xt)[NV], double (&yl)[NV],
-                                         double (&yt)[NV], double igh, const Net* net, double* ws, int lane) {
+                                         double (&yt)[NV], double igh, NetP net, double* ws, int lane) {
   double* XV = ws + L_XV;
   wave_sync();
 #pragma unroll
@@ -78,10 +78,10 @@ __device__ __forceinline__ void m0_times(const double (&xl)[NV], const double (&
     if (lane == 0) XV[v * (NS + 1) + NS] = 0.0;
   }
   net = opaque(net);
-  const u64* rxe = net->rxe;
-  const int* rf = net->rf;
-  const u32* svw = net->svw;
-  const u32* rowp = net->rowp;
+  const gptr<u64> rxe = net->rxe;
+  const gptr<int> rf = net->rf;
+  const gptr<u32> svw = net->svw;
+  const gptr<u32> rowp = net->rowp;
   const double* DV = ws + L_DV;
   double* V = ws + L_V;
   double* part = ws + L_PART;
@@ -153,7 +153,7 @@ __device__ __forceinline__ void m0_times(const double (&xl)[NV], const double (&
 // chemistry) the refinement stalls or diverges, and the try then goes to FP64.
 template <int NV>
 __device__ __forceinline__ bool msolve(double (&yl)[NV], double (&yt)[NV], const float (&c)[NT], const LeadRow& a21,
-                                       double dl, float dt, double igh, double wl, double wt, const Net* net,
+                                       double dl, float dt, double igh, double wl, double wt, NetP net,
                                        double* ws, int lane) {
   double bl[NV], bt[NV];
 #pragma unroll

@@ -156,7 +156,7 @@ __device__ __forceinline__ void from_lds(const double* y, double& vl, double& vt
 // barrier for each of its dense factorizations (cmd 1: factor W, 0: the workgroup's cells are done), so every
 // wavefront passes the same barriers.
 extern "C" __global__ void __launch_bounds__(WAVE* REC_WAVES)
-    mxc_recover(const Net* __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
+    mxc_recover(NetP __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
                 const double* __restrict__ Pa, const double* __restrict__ dta, const double* __restrict__ hin,
                 double* __restrict__ xout, double* __restrict__ hio, int* __restrict__ doneo,
                 double* __restrict__ noko, int* __restrict__ redo, const int* __restrict__ list,
@@ -198,7 +198,7 @@ extern "C" __global__ void __launch_bounds__(WAVE* REC_WAVES)
     const double m = fmin(h0, dt);
     int ntry = 0;
     for (int pass = 0;; pass++) {
-      const int* perm = opaque(net)->perm;
+      const gptr<int> perm = opaque(net)->perm;
       double Nl = lane < K0 ? xr[perm[lane]] : 0.0, Nt = lane < NT ? xr[perm[K0 + lane]] : 0.0;
       const double start = pass == 0 ? m : fmin(m, dt * pow(REC_RETRY_SPLIT, (double)pass));
       Step s = {fmax(start, H_MIN), 0.0, 0.0, 0.0, 0.0, false};

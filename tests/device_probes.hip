@@ -5,11 +5,11 @@
 // holds it) checked against its input x0[b]; the written result goes to xout[b] and the guard's bits to code[b].
 // One wavefront per cell.
 extern "C" __global__ void __launch_bounds__(WAVE)
-    mxc_guard_probe(const Net* __restrict__ net, const double* __restrict__ x0, const double* __restrict__ xres,
+    mxc_guard_probe(NetP __restrict__ net, const double* __restrict__ x0, const double* __restrict__ xres,
                     const int* __restrict__ done, const double* __restrict__ nok, double* __restrict__ xout,
                     int* __restrict__ code, double atol) {
   const int lane = (int)threadIdx.x, cell = (int)blockIdx.x;
-  const int* perm = net->perm;
+  const gptr<int> perm = net->perm;
   const double* xr = xres + (size_t)cell * NS;
   const double Nl = lane < K0 ? xr[perm[lane]] : 0.0, Nt = lane < NT ? xr[perm[K0 + lane]] : 0.0;
   const int c = guard_wave(Nl, Nt, x0 + (size_t)cell * NS, xout + (size_t)cell * NS, done[cell] != 0, nok[cell],

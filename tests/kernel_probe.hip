@@ -3,7 +3,7 @@
 // M0^-1 F, K1, K2, K3, K4. One workgroup of one wavefront.
 
 extern "C" __global__ void __launch_bounds__(WAVE)
-    mxc_probe(const Net* __restrict__ net, const double* __restrict__ xr, double Tc, double Pc, double ha,
+    mxc_probe(NetP __restrict__ net, const double* __restrict__ xr, double Tc, double Pc, double ha,
               const double* __restrict__ Jc, double* __restrict__ out) {
   extern __shared__ double lds[];
   double* ws = lds;
@@ -18,7 +18,7 @@ extern "C" __global__ void __launch_bounds__(WAVE)
   double c[NT];
   LeadRow a21;
   cell_rates(ws, ws + L_KF, Tc, nd, net, Jc, lane);
-  const int* perm = net->perm;
+  const gptr<int> perm = net->perm;
   double Nl = lane < K0 ? xr[perm[lane]] : 0.0, Nt = lane < NT ? xr[perm[K0 + lane]] : 0.0;
   double Fl = 0.0, Ft = 0.0, ul = 0.0, ut = 0.0, u0l = 0.0, u0t = 0.0;
   const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, ws + L_KF, lane);

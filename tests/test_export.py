@@ -49,7 +49,7 @@ def test_argument_lists_agree_with_the_kernels():
 def test_net_fields_are_the_kernels_struct():
     text = (_kernels.KERNEL_DIR / "common.hip").read_text()
     body = re.search(r"struct Net \{(.*?)\};", text, re.S).group(1)
-    assert re.findall(r"\*\s*(\w+);", body) == list(_kernels.NET_FIELDS)
+    assert re.findall(r"gptr<\w+>\s+(\w+);", body) == list(_kernels.NET_FIELDS)
 
 
 @pytest.fixture(scope="module")

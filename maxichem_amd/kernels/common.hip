@@ -42,6 +42,18 @@ __device__ __forceinline__ double readlane(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// The LLVM intrinsic behind v_writelane_b32 (clang has no builtin for it): old with lane `lane` set to src, a
+// wave-uniform value.
+__device__ int writelane_i32(int src, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
+// v with lane l's value replaced by x, a wave-uniform value (v_writelane, both 32-bit halves).
+__device__ __forceinline__ double writelane(double v, double x, int l) {
+  const long long b = __double_as_longlong(v), xb = __double_as_longlong(x);
+  const int lo = writelane_i32((int)(xb & 0xffffffffll), l, (int)(b & 0xffffffffll));
+  const int hi = writelane_i32((int)(xb >> 32), l, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // v from another lane by a DPP pattern (both 32-bit halves), all lanes active.
 template <int CTRL>
 __device__ __forceinline__ double dpp(double v) {
@@ -605,32 +617,41 @@ __device__ __forceinline__ void solve_lead_forward(double (&yl)[NV], double (&yt
 template <int NV>
 __device__ __forceinline__ void solve_trailing(double (&yt)[NV], const double (&c)[NT], double dt, int lane) {
   // the trailing block's solves: at step k every lane takes lane k's final value by v_readlane and is updated with
-  // its factor entry in column k, unmasked; lane k's value is recorded before its own update spoils it (and the
-  // lanes already recorded are spoilt freely), so the chain from step to step is a readlane and an fma. pin keeps
-  // each step's record in its step: left free, the compiler sinks the records to the end and holds every step's
-  // value until then
+  // its factor entry in column k, unmasked; the value read, lane k's final one, is written into lane k of the record
+  // by v_writelane (before the update spoils lane k; the lanes already recorded are spoilt freely), so a step has no
+  // compare or select, and the chain from step to step is a readlane and an fma. pin keeps each step's record in its
+  // step: left free, the compiler sinks the records to the end and holds every step's value until then
   double rec[NV];
 #pragma unroll
-  for (int k = 0; k < NT; k++) {  // unit L, trailing block
-    const bool own = lane_after(lane, yt[0]) == k;
+  for (int v = 0; v < NV; v++) rec[v] = 0.0;  // lanes past NT keep 0
+  // (a step's writelanes come after its fmas: a written scalar must wait a cycle before v_writelane reads it, and a
+  // written vector register before v_readlane reads it, so this order needs no wait instructions)
+  double yk[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-      const double yk = readlane(yt[v], k);
-      rec[v] = own ? yt[v] : (k == 0 ? 0.0 : rec[v]);
-      if (k + 1 < NT) yt[v] = fma(-c[k], yk, yt[v]);
-      pin(rec[v], yt[v]);
-    }
+  for (int k = 0; k < NT; k++) {  // unit L, trailing block
+#pragma unroll
+    for (int v = 0; v < NV; v++) yk[v] = readlane(yt[v], k);
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      if (k + 1 < NT) yt[v] = fma(-c[k], yk[v], yt[v]);
+#pragma unroll
+    for (int v = 0; v < NV; v++) rec[v] = writelane(rec[v], yk[v], k);
+#pragma unroll
+    for (int v = 0; v < NV; v++) pin(rec[v], yt[v]);
   }
 #pragma unroll
-  for (int k = NT - 1; k >= 0; k--) {  // unit U' = U diag(1 / U_kk), trailing block
-    const bool own = lane_after(lane, rec[0]) == k;
+  for (int v = 0; v < NV; v++) yt[v] = 0.0;
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-      const double zk = readlane(rec[v], k);
-      yt[v] = own ? rec[v] : (k == NT - 1 ? 0.0 : yt[v]);
-      if (k > 0) rec[v] = fma(-c[k], zk, rec[v]);
-      pin(rec[v], yt[v]);
-    }
+  for (int k = NT - 1; k >= 0; k--) {  // unit U' = U diag(1 / U_kk), trailing block
+#pragma unroll
+    for (int v = 0; v < NV; v++) yk[v] = readlane(rec[v], k);
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      if (k > 0) rec[v] = fma(-c[k], yk[v], rec[v]);
+#pragma unroll
+    for (int v = 0; v < NV; v++) yt[v] = writelane(yt[v], yk[v], k);
+#pragma unroll
+    for (int v = 0; v < NV; v++) pin(rec[v], yt[v]);
   }
 #pragma unroll
   for (int v = 0; v < NV; v++) yt[v] *= dt;

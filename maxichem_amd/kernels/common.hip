@@ -23,6 +23,7 @@ constexpr int REFUSED_COLD = 1;
 constexpr int REFUSED_BAD_INPUT = 2;
 constexpr int REFUSED_HOT = 3;
 constexpr int BROADCAST_WINDOW = 32;  // v_readlane broadcasts of a pivot row in flight ahead of their updates
+constexpr int SCHUR_BATCH = 16;  // a lead row's entries read ahead of their Schur updates
 constexpr int SMALL_LEAD = 8;  // at most this many lead rows: their factorization and solves are unrolled
 typedef double LeadRow[K0 > 0 ? K0 : 1];  // a trailing row's lead columns, one row per lane in registers
 
@@ -448,18 +449,24 @@ __device__ __forceinline__ void jacobian(T (&c)[NT], LeadRow& a21, double Nl, do
     wave_sync();
     jac_stream(opaque(net)->chw + CH_OFF[ch] * WAVE, CH_ROUNDS[ch], DV, ws, lane);
     wave_sync();
+    // N_j for -Jn N as wave-uniform values from the exchange vector (same-address LDS reads, kept in their chunk:
+    // hoisted ahead, they would hold a register pair each)
+    double nj[CHUNK];
+#pragma unroll
+    for (int j = 0; j < CHUNK; j++) nj[j] = ch < NLCH ? xv[ch * CHUNK + j < K0 ? ch * CHUNK + j : 0] : xv[K0 + (t0 + j < NT ? t0 + j : 0)];
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int j = 0; j < CHUNK; j++) {
       if (ch < NLCH) {
         if (ch * CHUNK + j < K0) {
           const double v = stage[j * NT + lane];
           a21[ch * CHUNK + j] = v;
-          mt = fma(v, readlane(Nl, ch * CHUNK + j), mt);
+          mt = fma(v, nj[j], mt);
         }
       } else if (t0 + j < NT) {
         const double v = lane < NT ? stage[j * NT + lane] : 0.0;
         c[t0 + j] = (T)v;
-        mt = fma(v, readlane(Nt, t0 + j), mt);
+        mt = fma(v, nj[j], mt);
       }
     }
   }
@@ -485,11 +492,18 @@ __device__ __forceinline__ void add_diagonal(T (&c)[NT], double igh, double* ws,
 // updates.
 template <typename T>
 __device__ __forceinline__ void schur_row(T (&c)[NT], double a, const double* ws, int k, int lane) {
-  const double u = lane < NT ? ws[k * NS + K0 + lane] : 0.0;
+  // the row's trailing entries as wave-uniform values from same-address LDS reads (one instruction per pair of
+  // entries, where a v_readlane broadcast takes two per entry), SCHUR_BATCH of them issued ahead of their updates
+  const double* u = ws + k * NS + K0;
 #pragma unroll
-  for (int j = 0; j < NT; j++) {
-    if (j >= BROADCAST_WINDOW) tie(c[j], c[j - BROADCAST_WINDOW]);
-    c[j] = (T)fma(-a, readlane(u, j), (double)c[j]);
+  for (int j0 = 0; j0 < NT; j0 += SCHUR_BATCH) {
+    double b[SCHUR_BATCH];
+#pragma unroll
+    for (int i = 0; i < SCHUR_BATCH; i++) b[i] = j0 + i < NT ? u[j0 + i] : 0.0;
+#pragma unroll
+    for (int i = 0; i < SCHUR_BATCH; i++)
+      if (j0 + i < NT) c[j0 + i] = (T)fma(-a, b[i], (double)c[j0 + i]);
+    asm volatile("" ::: "memory");  // the next batch's reads stay after this one: hoisted, they hold registers
   }
 }
 

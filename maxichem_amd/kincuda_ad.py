@@ -1,13 +1,16 @@
-"""The tangent (JVP) of one solver call on the GPU: ADSolver, with the reference's call form (maxichem/kincuda_ad.py:
-ADSolver.jvp, JvpResult; docs/method.md, "Derivatives").
+"""The tangent (JVP) and adjoint (VJP) of one solver call on the GPU: ADSolver, with the reference's call forms
+(maxichem/kincuda_ad.py: ADSolver.jvp, ADSolver.vjp, JvpResult, VjpResult, Cotangents; docs/method.md,
+"Derivatives").
 
 Built: the frozen contract (adapt=False): the accepted step sizes held fixed, rejected tries dropped, no tangent
 through dt or h, and none of h out. The tangents of x, T, P and J enter; the call's result is CudaSolver.launch's,
 bit for bit (the same kernels' arithmetic in the same order). A cell the guard fails is recomputed by the recovery
 without a tangent: its tangent is NaN, its flag 1 or 2 as in launch.
 
-Not built (refused by name): the adaptive contract (adapt=True), the rate multipliers lm and the rate table rc with
-their tangents, and the adjoint (vjp).
+The adjoint (vjp) of the same map is assembled from the tangent's columns, one tangent call per input coordinate.
+
+Not built (refused by name): the adaptive contract (adapt=True), and the rate multipliers lm and the rate table rc
+with their derivatives.
 """
 
 from __future__ import annotations
@@ -22,7 +25,7 @@ from ._constants import ATOL, DEFAULT_MAX_STEPS, DEFAULT_RTOL, REC_STEPS, WAVE
 from ._solver import check_inputs, check_photolysis, check_tensor
 from .kincuda import CudaSolver
 
-__all__ = ["ADSolver", "JvpResult"]
+__all__ = ["ADSolver", "Cotangents", "JvpResult", "VjpResult"]
 DMAX = 64  # directions of one jvp call (the reference's bound on the leading axis)
 
 
@@ -36,6 +39,32 @@ class JvpResult(NamedTuple):
     steps: torch.Tensor
     dx: torch.Tensor
     dh: torch.Tensor
+
+
+class Cotangents(NamedTuple):
+    """The cotangents of ADSolver.vjp's inputs (the reference's Cotangents), float64, each per unit of its input: x
+    [B, S]; T, P, dt, h [B]; lm [B, R] and rc [R, 9] (None: not built here); J [B, branches] (None without
+    photolysis)."""
+
+    x: torch.Tensor
+    T: torch.Tensor
+    P: torch.Tensor
+    dt: torch.Tensor
+    h: torch.Tensor
+    lm: torch.Tensor | None
+    rc: torch.Tensor | None
+    J: torch.Tensor | None
+
+
+class VjpResult(NamedTuple):
+    """What ADSolver.vjp returns (the reference's VjpResult): x, h, done, steps as CudaSolver's call, and the inputs'
+    cotangents."""
+
+    x: torch.Tensor
+    h: torch.Tensor
+    done: torch.Tensor
+    steps: torch.Tensor
+    cotangents: Cotangents
 
 
 class ADSolver(CudaSolver):
@@ -137,5 +166,58 @@ class ADSolver(CudaSolver):
         self._status = status
         return xo, ho, done, nok, redo
 
-    def vjp(self, *args, **kwargs):
-        raise NotImplementedError("the adjoint (vjp) is not built here; the tangent (jvp, adapt=False) is")
+    def vjp(self, x, T, P, dt, h, gx, *, gh=None, lm=None, rc=None, adapt: bool = True, rtol: float = DEFAULT_RTOL,
+            atol: float = ATOL, max_steps: int = DEFAULT_MAX_STEPS, J=None) -> VjpResult:  # fmt: skip
+        """The call and its inputs' cotangents for the output cotangent gx [B, S] (frozen contract: pass adapt=False;
+        the reference's ADSolver.vjp, maxichem/kincuda_ad.py:457-556). Everything after gx is keyword only.
+
+        Built from the tangent by columns, not by a reverse pass: one tangent call per input coordinate (each of x's
+        S entries, T, P and each photolysis branch), every cell along the same unit direction at once, and each
+        cell's cotangent the dot product of gx with its column. It costs S + 2 + branches tangent calls, each a call
+        with its tangent. Cotangents of dt and h are zero under the frozen contract, as is gh's contribution; those
+        of lm and rc are not built (None). A cell the guard lists has NaN cotangents, its flag as launch's."""
+        self.flag = None
+        self._require_open()
+        for name, a in (("lm", lm), ("rc", rc)):
+            if a is not None:
+                raise NotImplementedError(f"{name}: rate multipliers and the rate table are not differentiated here")
+        if adapt:
+            raise NotImplementedError("adapt=True: only the frozen contract (adapt=False) is built")
+        B = check_inputs(self.dev, self.ns, x, T, P, dt, h, rtol, atol, max_steps)
+        check_tensor("gx", gx, (B, self.ns), self.dev)
+        if gh is not None:
+            check_tensor("gh", gh, (B,), self.dev)
+        with torch.cuda.device(self.dev):
+            x, T, P, dt, h, gx = (a.contiguous() for a in (x, T, P, dt, h, gx))
+            jx = check_photolysis(J, B, self.nj, self.dev)
+            zero = torch.zeros(B, dtype=torch.float64, device=self.dev)
+            col = torch.empty(B, self.ns, dtype=torch.float64, device=self.dev)
+            g = Cotangents(x=torch.empty_like(x), T=torch.empty_like(T), P=torch.empty_like(T), dt=zero.clone(),
+                           h=zero.clone(), lm=None, rc=None,
+                           J=torch.empty(B, self.nj, dtype=torch.float64, device=self.dev) if self.nj else None)  # fmt: skip
+            unit = torch.zeros(B, self.ns, dtype=torch.float64, device=self.dev)
+            unit_j = torch.zeros(B, self.nj, dtype=torch.float64, device=self.dev) if self.nj else None
+            ones = torch.ones(B, dtype=torch.float64, device=self.dev)
+            result = None
+
+            def column(dx, dT, dP, dJ):
+                nonlocal result
+                result = self._jvp_once(x, T, P, dt, h, jx, rtol, atol, max_steps, B, dx, dT, dP, dJ, col)
+                return (gx * col).sum(1)
+
+            for s in range(self.ns):  # x's coordinates
+                unit.zero_()
+                unit[:, s] = 1.0
+                g.x[:, s] = column(unit, None, None, None)
+            unit.zero_()
+            g.T.copy_(column(unit, ones, None, None))
+            g.P.copy_(column(unit, None, ones, None))
+            for b in range(self.nj):  # the photolysis branches
+                unit_j.zero_()
+                unit_j[:, b] = 1.0
+                g.J[:, b] = column(unit, None, None, unit_j)
+            xo, ho, done, nok, redo = result
+            self.total += self.count
+            self._refuse(x, T, P, dt, jx, self._status)
+        self.flag = redo
+        return VjpResult(xo, ho, done.bool(), nok, g)

@@ -7,7 +7,8 @@ tests/test_kernel_copies.py).
   differences converge (two step sizes agree), so they pin the tangent.
 - GPU: jvp's x, h, done and steps are CudaSolver.launch's bit for bit; its tangent along random directions of x, T,
   P and J matches the central differences to 1e-5 of each cell's largest tangent entry; several directions in one
-  call give each direction's tangent; the refusals of what is not built.
+  call give each direction's tangent; vjp (built from the tangent's columns) is its transpose by the dot-product
+  test; the refusals of what is not built.
 """
 
 import numpy as np
@@ -20,6 +21,10 @@ from maxichem_amd import _tables, standin, workloads
 CELLS = 48
 EPS = 1e-6  # the central differences' step along a direction (directions scaled to relative moves of 1e-2)
 TANGENT_TOLERANCE = 1e-5  # of a cell's largest tangent entry: above the differences' own error (about 1e-6)
+# <gx, jvp(v)> against <vjp(gx), v>, of the terms' sum: the tangent is linear in its direction up to the rounding of
+# its solves, whose systems have condition numbers up to 1e12 (measured 1.8e-12); a transposed or misplaced column
+# gives a mismatch of order 1
+DOT_TOLERANCE = 1e-9
 
 
 def case(seed: int = 5):
@@ -129,3 +134,39 @@ def test_directions_share_one_call_and_unbuilt_parts_are_refused(gpu):
                    lm=torch.zeros(CELLS, ad.nr, dtype=torch.float64, device="cuda"))  # fmt: skip
         with pytest.raises(ValueError, match="dx"):
             ad.jvp(*args, t(d["dx"][:, :5]), zero, zero, zero, zero, adapt=False, J=t(inp["J"]))
+
+
+@pytest.mark.gpu
+def test_vjp_is_the_transpose_of_jvp(gpu):
+    # the adjoint by columns of the tangent: for random directions v and output cotangents gx, <gx, jvp(v)> per cell
+    # equals <vjp(gx), v> (the reference's dot-product test between its tangent and adjoint), and the primal is the
+    # call's, bit for bit
+    from maxichem_amd.kincuda_ad import ADSolver
+
+    net, _, inp, d = case(seed=9)
+    args = [t(inp[k]) for k in ("x", "T", "P", "dt", "h")]
+    gx = np.random.default_rng(11).standard_normal((CELLS, len(net.species)))
+    zero = t(np.zeros(CELLS))
+    with ADSolver(net, "cuda", photo=True) as ad:
+        r = ad.jvp(*args, t(d["dx"]), t(d["dT"]), t(d["dP"]), zero, zero, adapt=False, J=t(inp["J"]),
+                   dJ=t(d["dJ"]))  # fmt: skip
+        v = ad.vjp(*args, t(gx), adapt=False, J=t(inp["J"]))
+        for name in ("x", "h", "done", "steps"):
+            assert torch.equal(getattr(v, name), getattr(r, name)), name
+        with pytest.raises(NotImplementedError, match="adapt=True"):
+            ad.vjp(*args, t(gx), J=t(inp["J"]))
+        with pytest.raises(NotImplementedError, match="lm"):
+            ad.vjp(*args, t(gx), adapt=False, J=t(inp["J"]), lm=torch.zeros(CELLS, ad.nr, dtype=torch.float64,
+                                                                             device="cuda"))  # fmt: skip
+    g = v.cotangents
+    assert g.lm is None and g.rc is None and not bool(g.dt.any()) and not bool(g.h.any())
+    dxg = r.dx.cpu().numpy()
+    lhs = (gx * dxg).sum(1)
+    terms = np.concatenate([g.x.cpu().numpy() * d["dx"], (g.T.cpu().numpy() * d["dT"])[:, None],
+                            (g.P.cpu().numpy() * d["dP"])[:, None], g.J.cpu().numpy() * d["dJ"]], axis=1)  # fmt: skip
+    rhs = terms.sum(1)
+    ok = np.isfinite(lhs) & np.isfinite(rhs)
+    scale = np.abs(terms).sum(1) + np.abs(gx * dxg).sum(1)
+    err = np.abs(lhs - rhs)[ok] / scale[ok]
+    print(f"{int(ok.sum())} of {CELLS} cells finite; largest dot-product mismatch {err.max():.2e} of the terms' sum")
+    assert ok.sum() >= CELLS * 3 // 4 and err.max() <= DOT_TOLERANCE

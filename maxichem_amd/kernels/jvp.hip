@@ -365,8 +365,71 @@ __device__ __forceinline__ void guard_tangent(double Nl, double Nt, double Ndl, 
   if (ht) out[it] = (ydt - yt / S[0] * S[1]) / S[0];
 }
 
-// The call and its frozen tangent: the call kernel's loop (no proposals, FP64) with the tangent carried through each
-// accepted try, the guard with its tangent. Inputs' tangents: x0d [B][NS], Td, Pd [B], Jd [B][NJ] (may be null).
+// The tangent of a try's error (the call's try_error) along the state's tangent Ndot, the new state's Ydot and K4's
+// K4dot: q = K4 / (atol + rtol max(|N|, |Y|)) per species, the RMS of q, and while no step of the call is accepted
+// the larger of RMS and max |q| (the max taken at its lowest lane). Wave-uniform.
+__device__ __forceinline__ double error_tangent(double Nl, double Nt, double Yl, double Yt, double K4l, double K4t,
+                                                double Ndl, double Ndt, double Ydl, double Ydt, double K4dl, double K4dt,
+                                                bool first, double en, double rtol, double atol, int lane) {
+  const bool hl = lane < K0, ht = lane < NT;
+  auto part = [&](double N, double Y, double K4, double Nd, double Yd, double K4d, double& q, double& qd) {
+    const double aN = fabs(N), aY = fabs(Y);
+    const double w = atol + rtol * fmax(aN, aY);
+    const double wd = rtol * (aN >= aY ? (N >= 0.0 ? Nd : -Nd) : (Y >= 0.0 ? Yd : -Yd));  // of |N| or |Y|
+    q = K4 / w;
+    qd = (K4d - q * wd) / w;
+  };
+  double ql, qld, qt, qtd;
+  part(Nl, Yl, K4l, Ndl, Ydl, K4dl, ql, qld);
+  part(Nt, Yt, K4t, Ndt, Ydt, K4dt, qt, qtd);
+  if (!isfinite(en)) return 0.0;
+  double S[2] = {(hl ? ql * ql : 0.0) + (ht ? qt * qt : 0.0), (hl ? ql * qld : 0.0) + (ht ? qt * qtd : 0.0)};
+  wave_sums(S);
+  const double er = sqrt(S[0] / NS);
+  const double erd = er > 0.0 ? S[1] / (NS * er) : 0.0;
+  if (!(first && isfinite(er))) return erd;
+  const double a = hl ? fabs(ql) : 0.0, b = ht ? fabs(qt) : 0.0;
+  const double mine = fmax(a, b), mined = a >= b ? (hl ? (ql >= 0.0 ? qld : -qld) : 0.0) : (qt >= 0.0 ? qtd : -qtd);
+  const double em = wave_max(mine);
+  if (!(em > er)) return erd;
+  const unsigned long long at = __ballot(mine == em);
+  return readlane(mined, __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(at)));
+}
+
+// The tangent of the call's step factor (step_factor, the same branches) along the error's tangent end, the previous
+// try's error and step tangents pend and phad, and the step's tangent had: the law 0.9 err^(-1/3), held beyond its
+// clamps, and the secant shrink after two rejections through its fitted order.
+__device__ __forceinline__ double step_factor_tangent(double en, double end, bool ok, bool prej, double pen,
+                                                      double pend, double pha, double phad, double ha, double had) {
+  const bool efin = isfinite(en);
+  double fac = efin ? STEP_SAFETY / cbrt(fmax(en, ERR_FLOOR)) : FAC_NONFINITE;
+  double facd = (efin && en > ERR_FLOOR) ? -fac * end / (3.0 * en) : 0.0;
+  const double lo = ok ? FACACC : FACREJ;
+  if (fac < lo || fac > FACMAX) facd = 0.0;
+  fac = fmin(fmax(fac, lo), FACMAX);
+  if (!ok && efin && prej && en > 1.0 && pen > 1.0 && pha > ha) {
+    const double L1 = log(pen / en), L2 = log(pha / ha);
+    const double pe = L1 / L2;
+    if (pe > SECANT_MIN_ORDER) {
+      const double ped = ((pend / pen - end / en) * L2 - L1 * (phad / pha - had / ha)) / (L2 * L2);
+      const double o = fmin(pe, SECANT_MAX_ORDER), od = pe < SECANT_MAX_ORDER ? ped : 0.0;
+      const double r = SECANT_TARGET / en, a = 1.0 / o;
+      const double v = pow(r, a);
+      facd = v > SECMIN ? v * (-od / (o * o) * log(r) - a * end / en) : 0.0;
+      fac = fmax(v, SECMIN);
+    } else {
+      fac = SECMIN;
+      facd = 0.0;
+    }
+  }
+  if (!ok && fac > FAC_AFTER_REJECT) facd = 0.0;
+  return facd;
+}
+
+// The call and its tangent: the call kernel's loop (no proposals, FP64) with the tangent carried through each
+// accepted try (adapt 0: the frozen contract), or through every try with the step control's tangents (adapt 1: the
+// adaptive contract, with the tangents of dt and h in dtd, hd, and of the result's h into hdo), then the guard with
+// its tangent. Inputs' tangents: x0d [B][NS], Td, Pd [B], Jd [B][NJ] (may be null).
 // Output: xd [B][NS], the tangent of the guard's result (network order); a cell the guard fails gets NaN there (the
 // recovery recomputes it without a tangent).
 extern "C" __global__ void __launch_bounds__(WAVE* CPW)
@@ -376,7 +439,8 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
             int* __restrict__ redo, int* __restrict__ list, int* __restrict__ count, double* __restrict__ tries,
             const double* __restrict__ Jin, int B, double rtol, double atol, int max_steps,
             const double* __restrict__ x0d, const double* __restrict__ Td, const double* __restrict__ Pd,
-            const double* __restrict__ Jd, double* __restrict__ xd) {
+            const double* __restrict__ Jd, double* __restrict__ xd, int adapt, const double* __restrict__ dtd,
+            const double* __restrict__ hd, double* __restrict__ hdo) {
   extern __shared__ double lds[];
   const int lane = (int)(threadIdx.x & (WAVE - 1)), wv = (int)(threadIdx.x / WAVE);
   double* ws = lds + wv * (L_SIZE + J_SIZE);
@@ -417,29 +481,53 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
     const double h0 = isfinite(hin) ? hin : dt;
     if (lane == 0 && !isfinite(hin)) atomicAdd(count + COUNT_NONFINITE_H, 1);
     Step s = {fmax(fmin(h0, dt), H_MIN), 0.0, 0.0, 0.0, 0.0, false};
+    // the adaptive contract's tangents of the step control (wave-uniform): of s.h, s.t, s.pen, s.pha; the first step
+    // min(h, dt) clamped at H_MIN takes the tangent of the side it takes
+    const bool ad = adapt != 0;
+    const double dtdc = (ad && dtd) ? dtd[cell] : 0.0;
+    double hdot = 0.0, tdot = 0.0, pendot = 0.0, phadot = 0.0;
+    if (ad) {
+      const double h0d = isfinite(hin) ? (hd ? hd[cell] : 0.0) : dtdc;
+      hdot = fmin(h0, dt) >= H_MIN ? (h0 <= dt ? h0d : dtdc) : 0.0;
+    }
     int it = (dt > 0.0 && st == 0) ? 0 : max_steps;
     int ntry = 0;
     while (it < max_steps && s.t < dt * CALL_END) {
       const double ha = fmin(s.h, dt - s.t);
+      const double had = ad ? (s.h <= dt - s.t ? hdot : dtdc - tdot) : 0.0;
       double K1l, K1t, K3l, K3t, K4l, K4t;
       StageKeep kp;
       double dtr;
       stages(c, a21, Nl, Nt, ha, rtol, atol, net, nd, ws, rk, lane, K1l, K1t, K3l, K3t, K4l, K4t, &kp, &dtr);
       bool ok = false;
       double Yl, Yt;
-      const double en = try_error(Nl, Nt, K1l, K1t, K3l, K3t, K4l, K4t, Yl, Yt, s.t == 0.0, rtol, atol, lane, ok);
-      if (ok) {
-        // the tangent of the accepted try, with its factors still in place
+      const bool first = s.t == 0.0;
+      const double en = try_error(Nl, Nt, K1l, K1t, K3l, K3t, K4l, K4t, Yl, Yt, first, rtol, atol, lane, ok);
+      double end = 0.0;  // the error's tangent (adaptive contract)
+      if (ok || ad) {
+        // the tangent of the try (every try under the adaptive contract: a rejected one sets the next step), with
+        // its factors still in place
         const double ih = 1.0 / ha;
         double Fel, Fet, Gl, Gt, Ql, Qt;  // Fdot(N); Jdot K_i (eps-del parts)
         double K1dl, K1dt, K2dl, K2dt, K3dl, K3dt, K4dl, K4dt;
         rhs_tangent<true>(Nl, Nt, Ndl, Ndt, K1l, K1t, Fel, Fet, Gl, Gt, net, nd, ndd, ts, rk, rkd, lane);
         K1dl = Fel + Gl;
         K1dt = Fet + Gt;
+        // adaptive: W = I / (gamma h) - J moves with h, W Kdot = bdot + Jdot K + hdot / (gamma h^2) K, and the stages'
+        // K / h terms give -hdot / h^2 K
+        const double sg = had * ih * ih / GAMMA, hq = had * ih * ih;
+        if (ad) {
+          K1dl = fma(sg, K1l, K1dl);
+          K1dt = fma(sg, K1t, K1dt);
+        }
         w_solve(K1dl, K1dt, c, a21, kp, dtr, ws, lane);
         rhs_tangent<true>(Nl, Nt, Ndl, Ndt, kp.K2l, kp.K2t, Ql, Qt, Gl, Gt, net, nd, ndd, ts, rk, rkd, lane);
         K2dl = fma(RODAS_C21 * ih, K1dl, Fel) + Gl;
         K2dt = fma(RODAS_C21 * ih, K1dt, Fet) + Gt;
+        if (ad) {
+          K2dl += sg * kp.K2l - hq * RODAS_C21 * K1l;
+          K2dt += sg * kp.K2t - hq * RODAS_C21 * K1t;
+        }
         w_solve(K2dl, K2dt, c, a21, kp, dtr, ws, lane);
         // stage 3: F at Y3 = N + 2 K1 along Ydot3 = Ndot + 2 Kdot1; Jdot K3 at N
         double Y3el = Ndl + 2.0 * K1dl, Y3et = Ndt + 2.0 * K1dt, dummyl, dummyt;
@@ -448,6 +536,10 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
         rhs_tangent<true>(Nl, Nt, Ndl, Ndt, K3l, K3t, dummyl, dummyt, Gl, Gt, net, nd, ndd, ts, rk, rkd, lane);
         K3dl = fma(ih, K1dl - K2dl, Ql) + Gl;
         K3dt = fma(ih, K1dt - K2dt, Qt) + Gt;
+        if (ad) {
+          K3dl += sg * K3l - hq * (K1l - kp.K2l);
+          K3dt += sg * K3t - hq * (K1t - kp.K2t);
+        }
         w_solve(K3dl, K3dt, c, a21, kp, dtr, ws, lane);
         // stage 4: F at Y4 = N + 2 K1 + K3 along Ndot + 2 Kdot1 + Kdot3; Jdot K4 at N
         rhs_tangent<false>(Nl + 2.0 * K1l + K3l, Nt + 2.0 * K1t + K3t, Y3el + K3dl, Y3et + K3dt, 0.0, 0.0, Ql, Qt,
@@ -455,17 +547,32 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
         rhs_tangent<true>(Nl, Nt, Ndl, Ndt, K4l, K4t, dummyl, dummyt, Gl, Gt, net, nd, ndd, ts, rk, rkd, lane);
         K4dl = fma(ih, fma(-RODAS_C43, K3dl, K1dl - K2dl), Ql) + Gl;
         K4dt = fma(ih, fma(-RODAS_C43, K3dt, K1dt - K2dt), Qt) + Gt;
+        if (ad) {
+          K4dl += sg * K4l - hq * fma(-RODAS_C43, K3l, K1l - kp.K2l);
+          K4dt += sg * K4t - hq * fma(-RODAS_C43, K3t, K1t - kp.K2t);
+        }
         w_solve(K4dl, K4dt, c, a21, kp, dtr, ws, lane);
-        // the new state's tangent: zero where the state is clamped at 0
         const double ydl = Ndl + 2.0 * K1dl + K3dl + K4dl, ydt = Ndt + 2.0 * K1dt + K3dt + K4dt;
-        Ndl = lane < K0 && Yl >= 0.0 ? ydl : 0.0;
-        Ndt = lane < NT && Yt >= 0.0 ? ydt : 0.0;
+        if (ad) end = error_tangent(Nl, Nt, Yl, Yt, K4l, K4t, Ndl, Ndt, ydl, ydt, K4dl, K4dt, first, en, rtol, atol, lane);
+        if (ok) {
+          // the new state's tangent: zero where the state is clamped at 0
+          Ndl = lane < K0 && Yl >= 0.0 ? ydl : 0.0;
+          Ndt = lane < NT && Yt >= 0.0 ? ydt : 0.0;
+        }
+      }
+      if (ok) {
         Nl = fmax(Yl, 0.0);
         Nt = fmax(Yt, 0.0);
         s.t += ha;
         s.nok += 1.0;
+        tdot += had;
       }
       const double fac = step_factor(en, ok, s.prej, s.pen, s.pha, ha);
+      if (ad) {
+        hdot = fma(had, fac, ha * step_factor_tangent(en, end, ok, s.prej, s.pen, pendot, s.pha, phadot, ha, had));
+        pendot = end;
+        phadot = had;
+      }
       s.h = ha * fac;
       s.prej = !ok;
       s.pen = en;
@@ -488,6 +595,7 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
     }
     if (lane == 0) {
       hio[cell] = s.h;
+      if (hdo) hdo[cell] = !ad ? 0.0 : (st == 0 && code == 0 ? hdot : DNAN);
       doneo[cell] = fin ? 1 : 0;
       noko[cell] = s.nok;
       if (tries) tries[cell] = (double)ntry;

@@ -2,15 +2,16 @@
 (maxichem/kincuda_ad.py: ADSolver.jvp, ADSolver.vjp, JvpResult, VjpResult, Cotangents; docs/method.md,
 "Derivatives").
 
-Built: the frozen contract (adapt=False): the accepted step sizes held fixed, rejected tries dropped, no tangent
-through dt or h, and none of h out. The tangents of x, T, P and J enter; the call's result is CudaSolver.launch's,
-bit for bit (the same kernels' arithmetic in the same order). A cell the guard fails is recomputed by the recovery
-without a tangent: its tangent is NaN, its flag 1 or 2 as in launch.
+Built: both of the reference's contracts. adapt=True (its default) differentiates the step control: every try's
+tangent, rejected ones included, the error's and the step factor's, so the tangents of dt and h enter and the
+result's h has one. adapt=False (the frozen contract) holds the accepted step sizes fixed: rejected tries drop out, no
+tangent through dt or h, and none of h out. The tangents of x, T, P and J enter under both. The call's result is
+CudaSolver.launch's, bit for bit (the same kernels' arithmetic in the same order). A cell the guard fails is
+recomputed by the recovery without a tangent: its tangents are NaN, its flag 1 or 2 as in launch.
 
 The adjoint (vjp) of the same map is assembled from the tangent's columns, one tangent call per input coordinate.
 
-Not built (refused by name): the adaptive contract (adapt=True), and the rate multipliers lm and the rate table rc
-with their derivatives.
+Not built (refused by name): the rate multipliers lm and the rate table rc with their derivatives.
 """
 
 from __future__ import annotations
@@ -94,18 +95,19 @@ class ADSolver(CudaSolver):
     def jvp(self, x, T, P, dt, h, dx, dT, dP, ddt, dh, *, lm=None, dlm=None, rc=None, drc=None, adapt: bool = True,
             rtol: float = DEFAULT_RTOL, atol: float = ATOL, max_steps: int = DEFAULT_MAX_STEPS, J=None,
             dJ=None) -> JvpResult:  # fmt: skip
-        """The call and its tangent (frozen contract: pass adapt=False). Everything after dh is keyword only.
+        """The call and its tangent. Everything after dh is keyword only.
 
         dx, dT, dP, ddt, dh: the inputs' tangents in their shapes, or with a leading axis of D <= 64 directions on
-        every one; dJ [B, branches] (or [D, B, branches]) with photolysis. The result's x, h, done and steps are
-        CudaSolver.launch's; dx the tangent of x; dh zero. Refusals and counts as launch's."""
+        every one; dJ [B, branches] (or [D, B, branches]) with photolysis. adapt=True (the reference's default)
+        differentiates the step control too: every try's tangent, the error's and the step factor's, so ddt and dh
+        enter and the result's h has a tangent; adapt=False holds the accepted steps fixed (ddt and dh do not enter,
+        the result's dh is zero). The result's x, h, done and steps are CudaSolver.launch's; dx, dh the tangents of x
+        and h (NaN for a cell the guard lists). Refusals and counts as launch's."""
         self.flag = None
         self._require_open()
         for name, a in (("lm", lm), ("dlm", dlm), ("rc", rc), ("drc", drc)):
             if a is not None:
                 raise NotImplementedError(f"{name}: rate multipliers and the rate table are not differentiated here")
-        if adapt:
-            raise NotImplementedError("adapt=True: only the frozen contract (adapt=False) is built")
         B = check_inputs(self.dev, self.ns, x, T, P, dt, h, rtol, atol, max_steps)
         if dx.dim() not in (2, 3):
             raise ValueError(f"dx: [B, S] or [D, B, S] expected, got {list(dx.shape)}")
@@ -127,24 +129,26 @@ class ADSolver(CudaSolver):
             jx = check_photolysis(J, B, self.nj, self.dev)
             dJ = dJ if self.nj else None
             tx = dx if not one else dx[None]
-            tT, tP = (None if a is None else (a if not one else a[None]) for a in (dT, dP))
+            tT, tP, tdt, th = (None if a is None else (a if not one else a[None]) for a in (dT, dP, ddt, dh))
             tJ = None if dJ is None else (dJ if not one else dJ[None])
             xd = torch.empty(D, B, self.ns, dtype=torch.float64, device=self.dev)
+            hd = torch.empty(D, B, dtype=torch.float64, device=self.dev)
             result = None
+
+            def at(a, d):
+                return None if a is None else a[d].contiguous()
+
             for d in range(D):  # each direction repeats the call, with the same bits
-                result = self._jvp_once(x, T, P, dt, h, jx, rtol, atol, max_steps, B, tx[d].contiguous(),
-                                        None if tT is None else tT[d].contiguous(),
-                                        None if tP is None else tP[d].contiguous(),
-                                        None if tJ is None else tJ[d].contiguous(), xd[d])  # fmt: skip
+                result = self._jvp_once(x, T, P, dt, h, jx, rtol, atol, max_steps, B, tx[d].contiguous(), at(tT, d),
+                                        at(tP, d), at(tJ, d), xd[d], adapt, at(tdt, d), at(th, d), hd[d])  # fmt: skip
             xo, ho, done, nok, redo = result
             self.total += self.count
             self._refuse(x, T, P, dt, jx, self._status)
         self.flag = redo
-        dxo = xd[0] if one else xd
-        dho = torch.zeros((*lead, B), dtype=torch.float64, device=self.dev)
-        return JvpResult(xo, ho, done.bool(), nok, dxo, dho)
+        return JvpResult(xo, ho, done.bool(), nok, xd[0] if one else xd, hd[0] if one else hd)
 
-    def _jvp_once(self, x, T, P, dt, h, jx, rtol, atol, max_steps, B, dx, dT, dP, dJ, xd):
+    def _jvp_once(self, x, T, P, dt, h, jx, rtol, atol, max_steps, B, dx, dT, dP, dJ, xd, adapt=False, ddt=None,
+                  dh=None, hd=None):  # fmt: skip
         xo, ho = torch.empty_like(x), h.clone()
         done, redo, lst, status = (torch.empty(B, dtype=torch.int32, device=self.dev) for _ in range(4))
         nok = torch.empty(B, dtype=torch.float64, device=self.dev)
@@ -154,7 +158,8 @@ class ADSolver(CudaSolver):
         stream = torch.cuda.current_stream(self.dev).cuda_stream
         args = [p(self._net), p(x), p(T), p(P), p(dt), p(ho), p(xo), p(done), p(nok), p(status), p(redo), p(lst),
                 p(self.count), p(self.tries), p(jx), ctypes.c_int(B), ctypes.c_double(rtol), ctypes.c_double(atol),
-                ctypes.c_int(max_steps), p(dx), p(dT), p(dP), p(dJ), p(xd)]  # fmt: skip
+                ctypes.c_int(max_steps), p(dx), p(dT), p(dP), p(dJ), p(xd), ctypes.c_int(int(bool(adapt))), p(ddt),
+                p(dh), p(hd)]  # fmt: skip
         if B:
             _hip.launch(self._jvp, max(1, min(B, self.max_blocks_jvp)), WAVE, self.lds_jvp, stream, args)
         # the cells the guard failed: the recovery's result, without a tangent (NaN from the tangent kernel)
@@ -168,30 +173,31 @@ class ADSolver(CudaSolver):
 
     def vjp(self, x, T, P, dt, h, gx, *, gh=None, lm=None, rc=None, adapt: bool = True, rtol: float = DEFAULT_RTOL,
             atol: float = ATOL, max_steps: int = DEFAULT_MAX_STEPS, J=None) -> VjpResult:  # fmt: skip
-        """The call and its inputs' cotangents for the output cotangent gx [B, S] (frozen contract: pass adapt=False;
-        the reference's ADSolver.vjp, maxichem/kincuda_ad.py:457-556). Everything after gx is keyword only.
+        """The call and its inputs' cotangents for the output cotangents gx [B, S] and gh [B] (optional), under
+        either contract as jvp (the reference's ADSolver.vjp, maxichem/kincuda_ad.py:457-556). Everything after gx is
+        keyword only.
 
         Built from the tangent by columns, not by a reverse pass: one tangent call per input coordinate (each of x's
-        S entries, T, P and each photolysis branch), every cell along the same unit direction at once, and each
-        cell's cotangent the dot product of gx with its column. It costs S + 2 + branches tangent calls, each a call
-        with its tangent. Cotangents of dt and h are zero under the frozen contract, as is gh's contribution; those
-        of lm and rc are not built (None). A cell the guard lists has NaN cotangents, its flag as launch's."""
+        S entries, T, P, each photolysis branch, and under adapt=True dt and h), every cell along the same unit
+        direction at once, and each cell's cotangent gx . dx + gh dh of its column. It costs S + 2 + branches (+ 2)
+        tangent calls, each a call with its tangent. Under the frozen contract the cotangents of dt and h are zero.
+        Those of lm and rc are not built (None). A cell the guard lists has NaN cotangents, its flag as launch's."""
         self.flag = None
         self._require_open()
         for name, a in (("lm", lm), ("rc", rc)):
             if a is not None:
                 raise NotImplementedError(f"{name}: rate multipliers and the rate table are not differentiated here")
-        if adapt:
-            raise NotImplementedError("adapt=True: only the frozen contract (adapt=False) is built")
         B = check_inputs(self.dev, self.ns, x, T, P, dt, h, rtol, atol, max_steps)
         check_tensor("gx", gx, (B, self.ns), self.dev)
         if gh is not None:
             check_tensor("gh", gh, (B,), self.dev)
         with torch.cuda.device(self.dev):
             x, T, P, dt, h, gx = (a.contiguous() for a in (x, T, P, dt, h, gx))
+            gh = None if gh is None else gh.contiguous()
             jx = check_photolysis(J, B, self.nj, self.dev)
             zero = torch.zeros(B, dtype=torch.float64, device=self.dev)
             col = torch.empty(B, self.ns, dtype=torch.float64, device=self.dev)
+            colh = torch.empty(B, dtype=torch.float64, device=self.dev)
             g = Cotangents(x=torch.empty_like(x), T=torch.empty_like(T), P=torch.empty_like(T), dt=zero.clone(),
                            h=zero.clone(), lm=None, rc=None,
                            J=torch.empty(B, self.nj, dtype=torch.float64, device=self.dev) if self.nj else None)  # fmt: skip
@@ -200,22 +206,27 @@ class ADSolver(CudaSolver):
             ones = torch.ones(B, dtype=torch.float64, device=self.dev)
             result = None
 
-            def column(dx, dT, dP, dJ):
+            def column(dx, dT=None, dP=None, dJ=None, ddt=None, dh=None):
                 nonlocal result
-                result = self._jvp_once(x, T, P, dt, h, jx, rtol, atol, max_steps, B, dx, dT, dP, dJ, col)
-                return (gx * col).sum(1)
+                result = self._jvp_once(x, T, P, dt, h, jx, rtol, atol, max_steps, B, dx, dT, dP, dJ, col, adapt, ddt,
+                                        dh, colh)  # fmt: skip
+                out = (gx * col).sum(1)
+                return out if gh is None or not adapt else out + gh * colh
 
             for s in range(self.ns):  # x's coordinates
                 unit.zero_()
                 unit[:, s] = 1.0
-                g.x[:, s] = column(unit, None, None, None)
+                g.x[:, s] = column(unit)
             unit.zero_()
-            g.T.copy_(column(unit, ones, None, None))
-            g.P.copy_(column(unit, None, ones, None))
+            g.T.copy_(column(unit, dT=ones))
+            g.P.copy_(column(unit, dP=ones))
             for b in range(self.nj):  # the photolysis branches
                 unit_j.zero_()
                 unit_j[:, b] = 1.0
-                g.J[:, b] = column(unit, None, None, unit_j)
+                g.J[:, b] = column(unit, dJ=unit_j)
+            if adapt:  # dt and h enter under the adaptive contract
+                g.dt.copy_(column(unit, ddt=ones))
+                g.h.copy_(column(unit, dh=ones))
             xo, ho, done, nok, redo = result
             self.total += self.count
             self._refuse(x, T, P, dt, jx, self._status)

@@ -1,10 +1,12 @@
-"""The tangent of a call under the frozen contract (ADSolver.jvp, adapt=False), against central differences of the
-oracle's frozen primal: the cell's accepted tries replayed at the same sizes from perturbed inputs (the reference
-checks its tangent the same way, maxichem tests/test_jvp.py, and its primal bitwise against the call,
-tests/test_kernel_copies.py).
+"""The tangent of a call (ADSolver.jvp) against central differences of the oracle: under the frozen contract
+(adapt=False) of its frozen primal, the cell's accepted tries replayed at the same sizes from perturbed inputs; under
+the adaptive contract (adapt=True) of its whole call, on the cells whose tries and accepted steps the perturbations
+leave unchanged (the reference checks its tangent the same way, maxichem tests/test_jvp.py, and its primal bitwise
+against the call, tests/test_kernel_copies.py).
 
 - CPU: the oracle's replay of a cell's accepted tries gives the call's result bit for bit, and its central
-  differences converge (two step sizes agree), so they pin the tangent.
+  differences converge (two step sizes agree), so they pin the frozen tangent; the whole call's central differences
+  converge likewise on the cells whose decisions they leave unchanged, so they pin the adaptive one.
 - GPU: jvp's x, h, done and steps are CudaSolver.launch's bit for bit; its tangent along random directions of x, T,
   P and J matches the central differences to 1e-5 of each cell's largest tangent entry; several directions in one
   call give each direction's tangent; vjp (built from the tangent's columns) is its transpose by the dot-product
@@ -25,6 +27,9 @@ TANGENT_TOLERANCE = 1e-5  # of a cell's largest tangent entry: above the differe
 # its solves, whose systems have condition numbers up to 1e12 (measured 1.8e-12); a transposed or misplaced column
 # gives a mismatch of order 1
 DOT_TOLERANCE = 1e-9
+# the same under the adaptive contract: the step's tangent runs through the error estimate, whose terms for trace
+# species are rounding-dominated (K4 near atol), so linearity holds to the rounding of those terms (measured 1.7e-6)
+DOT_TOLERANCE_ADAPTIVE = 1e-4
 
 
 def case(seed: int = 5):
@@ -80,6 +85,12 @@ def t(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
+def plain_solver(net):
+    from maxichem_amd.kincuda import CudaSolver
+
+    return CudaSolver(net, "cuda", photo=True)
+
+
 @pytest.mark.gpu
 def test_jvp_repeats_the_call_and_matches_the_frozen_differences(gpu):
     from maxichem_amd.kincuda import CudaSolver
@@ -127,8 +138,6 @@ def test_directions_share_one_call_and_unbuilt_parts_are_refused(gpu):
         assert torch.equal(both.dx[0], one[0]) and torch.equal(both.dx[1], one[1])
         ok = torch.isfinite(one[0]).all(1)
         assert torch.allclose(one[1][ok], -2.0 * one[0][ok], rtol=1e-9, atol=0.0)  # linear in the direction
-        with pytest.raises(NotImplementedError, match="adapt=True"):
-            ad.jvp(*args, t(d["dx"]), zero, zero, zero, zero, J=t(inp["J"]))
         with pytest.raises(NotImplementedError, match="lm"):
             ad.jvp(*args, t(d["dx"]), zero, zero, zero, zero, adapt=False, J=t(inp["J"]),
                    lm=torch.zeros(CELLS, ad.nr, dtype=torch.float64, device="cuda"))  # fmt: skip
@@ -137,36 +146,112 @@ def test_directions_share_one_call_and_unbuilt_parts_are_refused(gpu):
 
 
 @pytest.mark.gpu
-def test_vjp_is_the_transpose_of_jvp(gpu):
-    # the adjoint by columns of the tangent: for random directions v and output cotangents gx, <gx, jvp(v)> per cell
-    # equals <vjp(gx), v> (the reference's dot-product test between its tangent and adjoint), and the primal is the
-    # call's, bit for bit
+@pytest.mark.parametrize("adapt", [False, True], ids=["frozen", "adaptive"])
+def test_vjp_is_the_transpose_of_jvp(gpu, adapt):
+    # the adjoint by columns of the tangent: for random directions v and output cotangents (gx, gh),
+    # <gx, dx> + gh dh of jvp(v) per cell equals <vjp(gx, gh), v> (the reference's dot-product test between its
+    # tangent and adjoint), and the primal is the call's, bit for bit
     from maxichem_amd.kincuda_ad import ADSolver
 
-    net, _, inp, d = case(seed=9)
+    net, _, inp, d = adaptive_case(seed=9)
     args = [t(inp[k]) for k in ("x", "T", "P", "dt", "h")]
-    gx = np.random.default_rng(11).standard_normal((CELLS, len(net.species)))
-    zero = t(np.zeros(CELLS))
+    rng = np.random.default_rng(11)
+    gx, gh = rng.standard_normal((CELLS, len(net.species))), rng.standard_normal(CELLS) / inp["h"]
     with ADSolver(net, "cuda", photo=True) as ad:
-        r = ad.jvp(*args, t(d["dx"]), t(d["dT"]), t(d["dP"]), zero, zero, adapt=False, J=t(inp["J"]),
+        r = ad.jvp(*args, t(d["dx"]), t(d["dT"]), t(d["dP"]), t(d["ddt"]), t(d["dh"]), adapt=adapt, J=t(inp["J"]),
                    dJ=t(d["dJ"]))  # fmt: skip
-        v = ad.vjp(*args, t(gx), adapt=False, J=t(inp["J"]))
+        v = ad.vjp(*args, t(gx), gh=t(gh), adapt=adapt, J=t(inp["J"]))
         for name in ("x", "h", "done", "steps"):
             assert torch.equal(getattr(v, name), getattr(r, name)), name
-        with pytest.raises(NotImplementedError, match="adapt=True"):
-            ad.vjp(*args, t(gx), J=t(inp["J"]))
         with pytest.raises(NotImplementedError, match="lm"):
-            ad.vjp(*args, t(gx), adapt=False, J=t(inp["J"]), lm=torch.zeros(CELLS, ad.nr, dtype=torch.float64,
+            ad.vjp(*args, t(gx), adapt=adapt, J=t(inp["J"]), lm=torch.zeros(CELLS, ad.nr, dtype=torch.float64,
                                                                              device="cuda"))  # fmt: skip
     g = v.cotangents
-    assert g.lm is None and g.rc is None and not bool(g.dt.any()) and not bool(g.h.any())
-    dxg = r.dx.cpu().numpy()
-    lhs = (gx * dxg).sum(1)
+    assert g.lm is None and g.rc is None
+    if not adapt:
+        assert not bool(g.dt.any()) and not bool(g.h.any()) and not bool(r.dh.any())
+    dxg, dhg = r.dx.cpu().numpy(), r.dh.cpu().numpy()
+    left = np.concatenate([gx * dxg, (gh * dhg)[:, None]], axis=1)
+    lhs = left.sum(1)
     terms = np.concatenate([g.x.cpu().numpy() * d["dx"], (g.T.cpu().numpy() * d["dT"])[:, None],
-                            (g.P.cpu().numpy() * d["dP"])[:, None], g.J.cpu().numpy() * d["dJ"]], axis=1)  # fmt: skip
+                            (g.P.cpu().numpy() * d["dP"])[:, None], g.J.cpu().numpy() * d["dJ"],
+                            (g.dt.cpu().numpy() * d["ddt"])[:, None], (g.h.cpu().numpy() * d["dh"])[:, None]],
+                           axis=1)  # fmt: skip
     rhs = terms.sum(1)
     ok = np.isfinite(lhs) & np.isfinite(rhs)
-    scale = np.abs(terms).sum(1) + np.abs(gx * dxg).sum(1)
+    scale = np.abs(terms).sum(1) + np.abs(left).sum(1)
     err = np.abs(lhs - rhs)[ok] / scale[ok]
     print(f"{int(ok.sum())} of {CELLS} cells finite; largest dot-product mismatch {err.max():.2e} of the terms' sum")
-    assert ok.sum() >= CELLS * 3 // 4 and err.max() <= DOT_TOLERANCE
+    assert ok.sum() >= CELLS * 3 // 4 and err.max() <= (DOT_TOLERANCE_ADAPTIVE if adapt else DOT_TOLERANCE)
+
+
+ADAPT_EPS = (1e-6, 1e-5)  # two steps of the whole call's central differences along the directions
+ADAPT_TOLERANCE = 1e-4  # of a cell's largest entry; the differences of the two steps agree to 1e-5 on the cells used
+
+
+def adaptive_case(seed: int = 13):
+    """case() with directions of dt (up to 1 %) and of the first step h (up to 10 %) as well."""
+    net, orc, inp, d = case(seed)
+    rng = np.random.default_rng(seed + 1)
+    d["ddt"] = 0.01 * inp["dt"] * rng.uniform(-1, 1, CELLS)
+    d["dh"] = 0.1 * inp["h"] * rng.uniform(-1, 1, CELLS)
+    return net, orc, inp, d
+
+
+def adaptive_differences(orc, inp, d):
+    """The whole call's central differences of x and h at the two steps, and the cells whose tries, accepted steps
+    and flags every perturbed call keeps (there the call is smooth along the direction)."""
+    def call(e):
+        return orc.call(inp["x"] + e * d["dx"], inp["T"] + e * d["dT"], inp["P"] + e * d["dP"], inp["dt"] + e * d["ddt"],
+                        inp["h"] + e * d["dh"], J=inp["J"] + e * d["dJ"])  # fmt: skip
+
+    base = call(0.0)
+    keep = base["flag"] == 0
+    fd = []
+    for e in ADAPT_EPS:
+        up, down = call(e), call(-e)
+        for r in (up, down):
+            keep &= (r["tries"] == base["tries"]) & (r["steps"] == base["steps"]) & (r["flag"] == 0)
+        fd.append(((up["x"] - down["x"]) / (2 * e), (up["h"] - down["h"]) / (2 * e)))
+    return base, keep, fd
+
+
+def test_the_whole_calls_differences_converge():
+    net, orc, inp, d = adaptive_case()
+    base, keep, fd = adaptive_differences(orc, inp, d)
+    (x1, h1), (x2, h2) = fd
+    scale = np.abs(x1).max(1)
+    agree = np.abs(x1 - x2).max(1) <= 1e-5 * scale
+    print(f"{int(keep.sum())} of {CELLS} cells keep their decisions; on them the differences agree on "
+          f"{int((agree & keep).sum())}")  # fmt: skip
+    assert keep.sum() >= CELLS // 2 and (agree & keep).sum() >= keep.sum() * 3 // 4
+
+
+@pytest.mark.gpu
+def test_adaptive_tangent_matches_the_differences_of_the_call(gpu):
+    from maxichem_amd.kincuda_ad import ADSolver
+
+    net, orc, inp, d = adaptive_case()
+    base, keep, fd = adaptive_differences(orc, inp, d)
+    (x1, h1), (x2, h2) = fd
+    args = [t(inp[k]) for k in ("x", "T", "P", "dt", "h")]
+    with plain_solver(net) as plain:
+        want = plain.launch(*args, J=t(inp["J"]))
+        tries = plain.tries.cpu().numpy()
+    with ADSolver(net, "cuda", photo=True) as ad:
+        r = ad.jvp(*args, t(d["dx"]), t(d["dT"]), t(d["dP"]), t(d["ddt"]), t(d["dh"]), J=t(inp["J"]), dJ=t(d["dJ"]))
+    for name in ("x", "h", "steps"):
+        assert torch.equal(getattr(r, name), getattr(want, name)), name
+    dxg, dhg = r.dx.cpu().numpy(), r.dh.cpu().numpy()
+    scale = np.abs(x1).max(1)
+    use = keep & (np.abs(x1 - x2).max(1) <= 1e-5 * scale) & (tries == base["tries"]) & (want.steps.cpu().numpy() ==
+                                                                                        base["steps"])  # fmt: skip
+    err = np.abs(dxg - x1).max(1)[use] / scale[use]
+    # h's differences converge only away from the step law's kinks: a cell whose last factor sits at a clamp (h = 20
+    # dt) takes one side at +eps and the other at -eps, and its two steps' differences disagree; those are left out
+    use_h = use & (np.abs(h1 - h2) <= 1e-5 * np.abs(h1))
+    errh = np.abs(dhg - h1)[use_h] / np.abs(h1[use_h])
+    print(f"{int(use.sum())} of {CELLS} cells compared for x, {int(use_h.sum())} for h; largest tangent error of x "
+          f"{err.max():.2e}, of h {errh.max():.2e}")  # fmt: skip
+    assert use.sum() >= CELLS // 3 and use_h.sum() >= CELLS // 3
+    assert err.max() <= ADAPT_TOLERANCE and errh.max() <= ADAPT_TOLERANCE

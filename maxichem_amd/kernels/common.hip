@@ -449,24 +449,28 @@ __device__ __forceinline__ void jacobian(T (&c)[NT], LeadRow& a21, double Nl, do
     wave_sync();
     jac_stream(opaque(net)->chw + CH_OFF[ch] * WAVE, CH_ROUNDS[ch], DV, ws, lane);
     wave_sync();
-    // N_j for -Jn N as wave-uniform values from the exchange vector (same-address LDS reads, kept in their chunk:
-    // hoisted ahead, they would hold a register pair each)
+    // N_j for -Jn N as wave-uniform values: with few lead rows from the exchange vector (same-address LDS reads,
+    // kept in their chunk: hoisted ahead, they would hold a register pair each), else by v_readlane (with the lead
+    // columns in registers the LDS reads' registers make the kernel spill)
     double nj[CHUNK];
+    if constexpr (K0 <= SMALL_LEAD) {
 #pragma unroll
-    for (int j = 0; j < CHUNK; j++) nj[j] = ch < NLCH ? xv[ch * CHUNK + j < K0 ? ch * CHUNK + j : 0] : xv[K0 + (t0 + j < NT ? t0 + j : 0)];
-    asm volatile("" ::: "memory");
+      for (int j = 0; j < CHUNK; j++)
+        nj[j] = ch < NLCH ? xv[ch * CHUNK + j < K0 ? ch * CHUNK + j : 0] : xv[K0 + (t0 + j < NT ? t0 + j : 0)];
+      asm volatile("" ::: "memory");
+    }
 #pragma unroll
     for (int j = 0; j < CHUNK; j++) {
       if (ch < NLCH) {
         if (ch * CHUNK + j < K0) {
           const double v = stage[j * NT + lane];
           a21[ch * CHUNK + j] = v;
-          mt = fma(v, nj[j], mt);
+          mt = fma(v, K0 <= SMALL_LEAD ? nj[j] : readlane(Nl, ch * CHUNK + j), mt);
         }
       } else if (t0 + j < NT) {
         const double v = lane < NT ? stage[j * NT + lane] : 0.0;
         c[t0 + j] = (T)v;
-        mt = fma(v, nj[j], mt);
+        mt = fma(v, K0 <= SMALL_LEAD ? nj[j] : readlane(Nt, t0 + j), mt);
       }
     }
   }

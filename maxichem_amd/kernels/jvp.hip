@@ -1,8 +1,9 @@
-// The tangent (JVP) of a call under the frozen contract: the call's own tries, bit for bit (the call kernel's device
-// functions in the call kernel's order), and along them the tangent of the state through every accepted try, with
-// the accepted step sizes held fixed (a rejected try drops out), then through the guard's normalisation and share
-// correction. The inputs' tangents: x, T, P and J (the reference's tangent kernel, maxichem/kernels/jvp.cuh, and its
-// contract in docs/method.md, "Derivatives"; here only the frozen one).
+// The tangent (JVP) of a call: the call's own tries, bit for bit (the call kernel's device functions in the call
+// kernel's order), and along them the tangent of the state through every accepted try, then through the guard's
+// normalisation and share correction. Under the frozen contract the accepted step sizes are held fixed (a rejected
+// try drops out); under the adaptive one every try is carried with the tangent of its step size, through the error
+// estimate and the step factor, and dt and h have tangents too. The inputs' tangents: x, T, P and J, and dt and h
+// (the reference's tangent kernel, maxichem/kernels/jvp.cuh, and its contracts in docs/method.md, "Derivatives").
 //
 // An accepted try of step h from N solves W K_i = b_i with W = I / (gamma h) - dF/dN at N. Its tangent, with h fixed:
 //   W Kdot_i = bdot_i + Jdot K_i,
@@ -10,7 +11,8 @@
 // F(N + eps Ndot + del K_i; rates + eps ratesdot), and bdot_1 = Fdot(N), bdot_2 = Fdot(N) + 4 Kdot_1 / h,
 // bdot_3 = Fdot(Y3)[Ydot3] + (Kdot_1 - Kdot_2) / h, bdot_4 = Fdot(Y4)[Ydot4] + (Kdot_1 - Kdot_2 - 8/3 Kdot_3) / h.
 // The solves reuse the try's factors of M0 and its Sherman-Morrison vector. The state's tangent is then
-// Ndot + 2 Kdot_1 + Kdot_3 + Kdot_4 where the new state is not clamped at 0 (and 0 where it is).
+// Ndot + 2 Kdot_1 + Kdot_3 + Kdot_4 where the new state is not clamped at 0 (and 0 where it is). With h moving
+// (the adaptive contract) each stage adds hdot / (gamma h^2) K_i, and each K / h term of bdot its -hdot K / h^2.
 //
 // F is evaluated in hyper-dual numbers (value, eps, del, eps-del parts) reaction by reaction: n = nd Y / sum Y, every
 // rate a product of its coefficient and up to three n, S v by the rhs's lane streams, F = (sum Y / nd) S v.
@@ -367,7 +369,9 @@ __device__ __forceinline__ void guard_tangent(double Nl, double Nt, double Ndl, 
 
 // The tangent of a try's error (the call's try_error) along the state's tangent Ndot, the new state's Ydot and K4's
 // K4dot: q = K4 / (atol + rtol max(|N|, |Y|)) per species, the RMS of q, and while no step of the call is accepted
-// the larger of RMS and max |q| (the max taken at its lowest lane). Wave-uniform.
+// the larger of RMS and max |q|. Ties follow the reference's conventions (docs/method.md, "Derivatives"): half to
+// each side of a tie of min or max, a maximum over species shared evenly among those attaining it, d|a| = 0 at 0, and a
+// clamp passes at its bound. Wave-uniform.
 __device__ __forceinline__ double error_tangent(double Nl, double Nt, double Yl, double Yt, double K4l, double K4t,
                                                 double Ndl, double Ndt, double Ydl, double Ydt, double K4dl, double K4dt,
                                                 bool first, double en, double rtol, double atol, int lane) {
@@ -375,7 +379,9 @@ __device__ __forceinline__ double error_tangent(double Nl, double Nt, double Yl,
   auto part = [&](double N, double Y, double K4, double Nd, double Yd, double K4d, double& q, double& qd) {
     const double aN = fabs(N), aY = fabs(Y);
     const double w = atol + rtol * fmax(aN, aY);
-    const double wd = rtol * (aN >= aY ? (N >= 0.0 ? Nd : -Nd) : (Y >= 0.0 ? Yd : -Yd));  // of |N| or |Y|
+    // d|a| = 0 at a = 0, and a tie of the max takes half of each side's tangent
+    const double dN = N > 0.0 ? Nd : (N < 0.0 ? -Nd : 0.0), dY = Y > 0.0 ? Yd : (Y < 0.0 ? -Yd : 0.0);
+    const double wd = rtol * (aN > aY ? dN : (aN < aY ? dY : 0.5 * (dN + dY)));
     q = K4 / w;
     qd = (K4d - q * wd) / w;
   };
@@ -389,11 +395,15 @@ __device__ __forceinline__ double error_tangent(double Nl, double Nt, double Yl,
   const double erd = er > 0.0 ? S[1] / (NS * er) : 0.0;
   if (!(first && isfinite(er))) return erd;
   const double a = hl ? fabs(ql) : 0.0, b = ht ? fabs(qt) : 0.0;
-  const double mine = fmax(a, b), mined = a >= b ? (hl ? (ql >= 0.0 ? qld : -qld) : 0.0) : (qt >= 0.0 ? qtd : -qtd);
-  const double em = wave_max(mine);
-  if (!(em > er)) return erd;
-  const unsigned long long at = __ballot(mine == em);
-  return readlane(mined, __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(at)));
+  const double da = hl ? (ql > 0.0 ? qld : (ql < 0.0 ? -qld : 0.0)) : 0.0;
+  const double db = ht ? (qt > 0.0 ? qtd : (qt < 0.0 ? -qtd : 0.0)) : 0.0;
+  const double em = wave_max(fmax(a, b));
+  if (em < er) return erd;
+  // the max over the species shared evenly among those attaining it
+  double M[2] = {(a == em ? da : 0.0) + (b == em ? db : 0.0), (a == em ? 1.0 : 0.0) + (b == em ? 1.0 : 0.0)};
+  wave_sums(M);
+  const double emd = M[0] / M[1];
+  return em > er ? emd : 0.5 * (emd + erd);
 }
 
 // The tangent of the call's step factor (step_factor, the same branches) along the error's tangent end, the previous
@@ -403,7 +413,7 @@ __device__ __forceinline__ double step_factor_tangent(double en, double end, boo
                                                       double pend, double pha, double phad, double ha, double had) {
   const bool efin = isfinite(en);
   double fac = efin ? STEP_SAFETY / cbrt(fmax(en, ERR_FLOOR)) : FAC_NONFINITE;
-  double facd = (efin && en > ERR_FLOOR) ? -fac * end / (3.0 * en) : 0.0;
+  double facd = (efin && en >= ERR_FLOOR) ? -fac * end / (3.0 * en) : 0.0;
   const double lo = ok ? FACACC : FACREJ;
   if (fac < lo || fac > FACMAX) facd = 0.0;
   fac = fmin(fmax(fac, lo), FACMAX);
@@ -412,10 +422,10 @@ __device__ __forceinline__ double step_factor_tangent(double en, double end, boo
     const double pe = L1 / L2;
     if (pe > SECANT_MIN_ORDER) {
       const double ped = ((pend / pen - end / en) * L2 - L1 * (phad / pha - had / ha)) / (L2 * L2);
-      const double o = fmin(pe, SECANT_MAX_ORDER), od = pe < SECANT_MAX_ORDER ? ped : 0.0;
+      const double o = fmin(pe, SECANT_MAX_ORDER), od = pe <= SECANT_MAX_ORDER ? ped : 0.0;
       const double r = SECANT_TARGET / en, a = 1.0 / o;
       const double v = pow(r, a);
-      facd = v > SECMIN ? v * (-od / (o * o) * log(r) - a * end / en) : 0.0;
+      facd = v >= SECMIN ? v * (-od / (o * o) * log(r) - a * end / en) : 0.0;
       fac = fmax(v, SECMIN);
     } else {
       fac = SECMIN;
@@ -482,19 +492,21 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
     if (lane == 0 && !isfinite(hin)) atomicAdd(count + COUNT_NONFINITE_H, 1);
     Step s = {fmax(fmin(h0, dt), H_MIN), 0.0, 0.0, 0.0, 0.0, false};
     // the adaptive contract's tangents of the step control (wave-uniform): of s.h, s.t, s.pen, s.pha; the first step
-    // min(h, dt) clamped at H_MIN takes the tangent of the side it takes
+    // min(h, dt) clamped at H_MIN takes the tangent of the side it takes, half of each at a tie
     const bool ad = adapt != 0;
     const double dtdc = (ad && dtd) ? dtd[cell] : 0.0;
     double hdot = 0.0, tdot = 0.0, pendot = 0.0, phadot = 0.0;
     if (ad) {
       const double h0d = isfinite(hin) ? (hd ? hd[cell] : 0.0) : dtdc;
-      hdot = fmin(h0, dt) >= H_MIN ? (h0 <= dt ? h0d : dtdc) : 0.0;
+      const double md = h0 < dt ? h0d : (h0 > dt ? dtdc : 0.5 * (h0d + dtdc)), m = fmin(h0, dt);
+      hdot = m > H_MIN ? md : (m < H_MIN ? 0.0 : 0.5 * md);  // the first step max(m, H_MIN)
     }
     int it = (dt > 0.0 && st == 0) ? 0 : max_steps;
     int ntry = 0;
     while (it < max_steps && s.t < dt * CALL_END) {
       const double ha = fmin(s.h, dt - s.t);
-      const double had = ad ? (s.h <= dt - s.t ? hdot : dtdc - tdot) : 0.0;
+      const double rem = dt - s.t;
+      const double had = ad ? (s.h < rem ? hdot : (s.h > rem ? dtdc - tdot : 0.5 * (hdot + dtdc - tdot))) : 0.0;
       double K1l, K1t, K3l, K3t, K4l, K4t;
       StageKeep kp;
       double dtr;

@@ -228,8 +228,10 @@ __device__ __forceinline__ int species_of(u64 w, int m) { return (int)((w >> (8 
 // The rate coefficients of every reaction at the cell's T and nd into rk (kf [NR] then kr [NR]; LDS, or the wave's
 // slot of global scratch), kept there for the call.
 // G / RT (elimination order) goes to the scratch at L_UNION first.
+// lmc: the cell's log rate multipliers [NR] or null; each reaction's two coefficients are scaled by exp(lm) after the
+// collision limit, a photolysis branch's kf = J exp(lm) (the reference's tangent kernels' rates, ad_common.cuh:136-168)
 __device__ __forceinline__ void cell_rates(double* ws, double* rk, double T, double nd, NetP net,
-                                           const double* Jc, int lane) {
+                                           const double* Jc, int lane, const double* lmc = nullptr) {
   net = opaque(net);
   const gptr<double> rc = net->rc;
   const gptr<int> rf = net->rf;
@@ -266,6 +268,11 @@ __device__ __forceinline__ void cell_rates(double* ws, double* rk, double T, dou
         f *= c;
         b *= c;
       }
+    }
+    if (lmc) {
+      const double e = exp(lmc[r]);
+      f *= e;
+      b *= e;
     }
     kf[r] = f;
     kr[r] = b;

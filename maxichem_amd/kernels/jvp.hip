@@ -2,8 +2,9 @@
 // kernel's order), and along them the tangent of the state through every accepted try, then through the guard's
 // normalisation and share correction. Under the frozen contract the accepted step sizes are held fixed (a rejected
 // try drops out); under the adaptive one every try is carried with the tangent of its step size, through the error
-// estimate and the step factor, and dt and h have tangents too. The inputs' tangents: x, T, P and J, and dt and h
-// (the reference's tangent kernel, maxichem/kernels/jvp.cuh, and its contracts in docs/method.md, "Derivatives").
+// estimate and the step factor, and dt and h have tangents too. The inputs' tangents: x, T, P, J, the log rate
+// multipliers lm and the rate table's Arrhenius columns, and dt and h (the reference's tangent kernel,
+// maxichem/kernels/jvp.cuh, and its contracts in docs/method.md, "Derivatives").
 //
 // An accepted try of step h from N solves W K_i = b_i with W = I / (gamma h) - dF/dN at N. Its tangent, with h fixed:
 //   W Kdot_i = bdot_i + Jdot K_i,
@@ -40,12 +41,16 @@ __device__ __forceinline__ Dual dexp10(Dual a) {
   return {e, e * a.d * 2.302585092994045684};
 }
 
-// The rate law of row q at T (with its tangent) and nd (with its tangent): the call's law() in dual numbers.
-__device__ __forceinline__ Dual law_dual(const double* __restrict__ q, int fl, Dual T, Dual lT, Dual iT, Dual nd) {
-  Dual k = dexp(Dual{q[0], 0.0} + q[1] * lT - q[2] * iT);
-  if (fl & FL_SECOND_K0) k = k + dexp(Dual{q[3], 0.0} + q[4] * lT - q[5] * iT);
+// The rate law of row q at T (with its tangent) and nd (with its tangent): the call's law() in dual numbers. qd: the
+// tangents of the row's nine Arrhenius columns (ln A, B, C of k0, of the second k0 and of k_inf), or null; the Troe
+// parameters and the efficiency have none.
+__device__ __forceinline__ Dual law_dual(gptr<double> __restrict__ q, const double* __restrict__ qd, int fl, Dual T,
+                                         Dual lT, Dual iT, Dual nd) {
+  auto a = [&](int j) { return Dual{q[j], qd ? qd[j] : 0.0}; };
+  Dual k = dexp(a(0) + a(1) * lT - a(2) * iT);
+  if (fl & FL_SECOND_K0) k = k + dexp(a(3) + a(4) * lT - a(5) * iT);
   if (fl & FL_FALLOFF) {
-    const Dual kinf = dexp(Dual{q[6], 0.0} + q[7] * lT - q[8] * iT);
+    const Dual kinf = dexp(a(6) + a(7) * lT - a(8) * iT);
     const Dual pr = k * nd / kinf;
     k = k.v > 0.0 ? k / (Dual{1.0, 0.0} + pr) : Dual{0.0, 0.0};
 #if HAS_TROE
@@ -73,11 +78,14 @@ __device__ __forceinline__ Dual law_dual(const double* __restrict__ q, int fl, D
   return k;
 }
 
-// The tangents of every reaction's coefficients (kfdot, krdot into rkd [2 NR]) from the tangents of T, nd and J (Jd).
-// The coefficients follow the call's cell_rates: G / RT from NASA-9, whose T-derivative is -H / (R T^2); the
-// collision limit's factor takes the tangent of the larger direction (half of each at a tie).
-__device__ __forceinline__ void rates_tangent(double* ts, double* rkd, double T, double Td, double nd, double ndd, NetP net,
-                              const double* Jdc, int lane) {
+// The tangents of every reaction's coefficients (kfdot, krdot into rkd [2 NR]) from the tangents of T, nd, J (Jdc),
+// the log multipliers (lmc and their tangents lmdc, each null or [NR]) and the rate table's Arrhenius columns (rcd
+// [NR][9] or null). The coefficients follow the call's cell_rates: G / RT from NASA-9, whose T-derivative is
+// -H / (R T^2); the collision limit KCOLL / q of the larger direction q takes q's tangent (half of each direction's at a
+// tie of the two, and half of it when q is at the limit); then both directions times exp(lm).
+__device__ __forceinline__ void rates_tangent(double* ts, double* rkd, double T, double Td, double nd, double ndd,
+                                              NetP net, const double* Jc, const double* Jdc, const double* lmc,
+                                              const double* lmdc, const double* rcd, int lane) {
   net = opaque(net);
   const gptr<double> rc = net->rc;
   const gptr<int> rf = net->rf;
@@ -102,10 +110,10 @@ __device__ __forceinline__ void rates_tangent(double* ts, double* rkd, double T,
     const int fl = rf[r];
     Dual f{0.0, 0.0}, b{0.0, 0.0};
     if (fl & FL_PHOTO) {
-      f = Dual{0.0, Jdc ? Jdc[r - NRT] : 0.0};
+      f = Dual{Jc[r - NRT], Jdc ? Jdc[r - NRT] : 0.0};
     } else {
       const u64 w = rxe[r];
-      const Dual k = law_dual(rc + RCW * r, fl, Tq, lTq, iTq, ndq);
+      const Dual k = law_dual(rc + RCW * r, rcd ? rcd + 9 * r : nullptr, fl, Tq, lTq, iTq, ndq);
       const int s0 = species_of(w, 0), s1 = species_of(w, 1), s2 = species_of(w, 2);
       const int s3 = species_of(w, 3), s4 = species_of(w, 4), s5 = species_of(w, 5);
       const int mf = (fl >> FL_NF_SHIFT) & 3, mr = (fl >> FL_NR_SHIFT) & 3;
@@ -118,12 +126,19 @@ __device__ __forceinline__ void rates_tangent(double* ts, double* rkd, double T,
         return m >= 2 ? kk * (m == 3 ? ndq : Dual{1.0, 0.0}) : (tb ? kk / ndq : Dual{0.0, 0.0});
       };
       const Dual qf = bim(mf, f), qr = bim(mr, b);
-      const Dual qb = qf.v > qr.v ? qf : (qr.v > qf.v ? qr : Dual{qf.v, 0.5 * (qf.d + qr.d)});
-      if (qb.v > KCOLL) {
+      Dual qb = qf.v > qr.v ? qf : (qr.v > qf.v ? qr : Dual{qf.v, 0.5 * (qf.d + qr.d)});
+      if (qb.v >= KCOLL) {  // at the limit itself the factor is 1 and takes half of its tangent
+        if (qb.v == KCOLL) qb.d *= 0.5;
         const Dual c = Dual{KCOLL, 0.0} / qb;
         f = f * c;
         b = b * c;
       }
+    }
+    if (lmc || lmdc) {  // a tangent of lm without lm: at lm = 0
+      const double e = lmc ? exp(lmc[r]) : 1.0;
+      const Dual m{e, lmdc ? e * lmdc[r] : 0.0};
+      f = f * m;
+      b = b * m;
     }
     rkd[r] = f.d;
     rkd[NR + r] = b.d;
@@ -450,7 +465,8 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
             const double* __restrict__ Jin, int B, double rtol, double atol, int max_steps,
             const double* __restrict__ x0d, const double* __restrict__ Td, const double* __restrict__ Pd,
             const double* __restrict__ Jd, double* __restrict__ xd, int adapt, const double* __restrict__ dtd,
-            const double* __restrict__ hd, double* __restrict__ hdo) {
+            const double* __restrict__ hd, double* __restrict__ hdo, const double* __restrict__ lm,
+            const double* __restrict__ lmd, const double* __restrict__ rcd) {
   extern __shared__ double lds[];
   const int lane = (int)(threadIdx.x & (WAVE - 1)), wv = (int)(threadIdx.x / WAVE);
   double* ws = lds + wv * (L_SIZE + J_SIZE);
@@ -474,6 +490,9 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
     double pm = fmax(lane < K0 ? Nl : 0.0, lane < NT ? Nt : 0.0);
     const double* Jc = NJ ? Jin + (size_t)cell * NJ : nullptr;
     for (int b = lane; b < NJ; b += WAVE) p += Jc[b] >= 0.0 ? 0.0 * Jc[b] : DNAN;
+    const double* lmc = lm ? lm + (size_t)cell * NR : nullptr;
+    if (lmc)  // a non-finite log multiplier makes the cell bad input
+      for (int r = lane; r < NR; r += WAVE) p += 0.0 * lmc[r];
     const double ps = wave_sum(p), pmax = wave_max(pm);
     const bool fine = fabs(ps - 1.0) <= XSUMTOL && pmax > 0.0 && isfinite(Tc) && isfinite(nd) && nd > 0.0 &&
                       isfinite(dt) && dt >= 0.0;
@@ -484,8 +503,9 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
     const double Tdc = Td ? Td[cell] : 0.0, Pdc = Pd ? Pd[cell] : 0.0;
     const double ndd = nd * (Pdc / Pa[cell] - Tdc / Tc);
     if (st == 0) {
-      cell_rates(ws, rk, Tc, nd, net, Jc, lane);
-      rates_tangent(ts, rkd, Tc, Tdc, nd, ndd, net, (NJ && Jd) ? Jd + (size_t)cell * NJ : nullptr, lane);
+      cell_rates(ws, rk, Tc, nd, net, Jc, lane, lmc);
+      rates_tangent(ts, rkd, Tc, Tdc, nd, ndd, net, Jc, (NJ && Jd) ? Jd + (size_t)cell * NJ : nullptr, lmc,
+                    lmd ? lmd + (size_t)cell * NR : nullptr, rcd, lane);
     }
     const double hin = hio[cell];
     const double h0 = isfinite(hin) ? hin : dt;
@@ -624,5 +644,76 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
     int nx = 0;
     if (lane == 0) nx = waves + atomicAdd(count + COUNT_QUEUE, 1);
     q = __builtin_amdgcn_readfirstlane(nx);
+  }
+}
+
+// The recovery of the cells the tangent kernel lists, with the cells' log rate multipliers lm [B][NR] (the call's
+// recovery otherwise, bit for bit).
+extern "C" __global__ void __launch_bounds__(WAVE* REC_WAVES)
+    mxc_recover_lm(NetP __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
+                   const double* __restrict__ Pa, const double* __restrict__ dta, const double* __restrict__ hin,
+                   double* __restrict__ xout, double* __restrict__ hio, int* __restrict__ doneo,
+                   double* __restrict__ noko, int* __restrict__ redo, const int* __restrict__ list,
+                   int* __restrict__ count, double* __restrict__ tries, const double* __restrict__ Jin, double rtol,
+                   double atol, int max_steps, double* __restrict__ rscratch, const double* __restrict__ lm) {
+  recover_cells(net, x0, Ta, Pa, dta, hin, xout, hio, doneo, noko, redo, list, count, tries, Jin, rtol, atol,
+                max_steps, rscratch, lm);
+}
+
+// The rate table's cotangent grc [NR][9] (its nine Arrhenius columns) from the cells' cotangents of their log rate
+// multipliers, glm [B][NR]. A multiplier and the law enter a reaction's two coefficients alike: d ln kf = d ln kr =
+// (1 - wA) d ln k + d lm, with k the law before the collision limit and wA the limit's share (1 above the limit, 1/2
+// at it, 0 below: the limit's factor KCOLL / q moves with k). So grc[r][j] = sum over the cells of
+// (1 - wA) (d ln k / d rc[r][j]) glm[cell][r], each lane's cells in index order and the lanes' shares by a fixed-order
+// wave sum, the same bits on every run. Cells with skip[cell] != 0 (the guard's still-failed ones, whose cotangents
+// are NaN) are left out. One wavefront per reaction; a photolysis branch has no law (0).
+extern "C" __global__ void __launch_bounds__(WAVE)
+    mxc_rc_cotangent(NetP __restrict__ net, const double* __restrict__ Ta, const double* __restrict__ Pa,
+                     const double* __restrict__ glm, const int* __restrict__ skip, int B, double* __restrict__ grc) {
+  const int r = (int)blockIdx.x, lane = (int)threadIdx.x;
+  if (r >= NR) return;
+  const NetP nt = opaque(net);
+  const int fl = nt->rf[r];
+  double acc[9];
+#pragma unroll
+  for (int j = 0; j < 9; j++) acc[j] = 0.0;
+  if (!(fl & FL_PHOTO)) {
+    const gptr<double> q = nt->rc + RCW * r;
+    const u64 w = nt->rxe[r];
+    const int mf = (fl >> FL_NF_SHIFT) & 3, mr = (fl >> FL_NR_SHIFT) & 3;
+    const bool tb = (fl & FL_THREE_BODY) != 0;
+    for (int cell = lane; cell < B; cell += WAVE) {
+      const double gl = glm[(size_t)cell * NR + r];
+      if ((skip && skip[cell] != 0) || gl == 0.0) continue;
+      const double T = Ta[cell], nd = Pa[cell] / (KBOLTZ * T);
+      const double lT = log(T), iT = 1.0 / T, lKT = log(KBOLTZ * T / P0BAR);
+      const gptr<double> cf = nt->nasa + (T < NASA_SPLIT_T ? 0 : NS * 9);
+      double dg = 0.0;
+#pragma unroll
+      for (int m = 0; m < 6; m++) {
+        const int sp = species_of(w, m);
+        const double gs = sp < NS ? gibbs_rt(cf + 9 * sp, T, lT, iT) : 0.0;
+        dg += m < 3 ? -gs : gs;
+      }
+      const double k = law(q, fl, T, lT, iT, nd), c3 = tb ? nd : 1.0;
+      const double f = k * c3, b = (k * exp(dg + (double)(mr - mf) * lKT)) * c3;
+      const double qb = fmax(bimolecular(mf, tb, f, nd), bimolecular(mr, tb, b, nd));
+      const double share = qb > KCOLL ? 0.0 : (qb == KCOLL ? 0.5 : 1.0);  // 1 - wA
+      if (share == 0.0 || !(k > 0.0)) continue;  // no law derivative left (k = 0: no rate to scale)
+      const Dual Tq{T, 0.0}, lTq{lT, 0.0}, iTq{iT, 0.0}, ndq{nd, 0.0};
+#pragma unroll
+      for (int j = 0; j < 9; j++) {
+        double unit[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) unit[i] = i == j ? 1.0 : 0.0;
+        const Dual kd = law_dual(q, unit, fl, Tq, lTq, iTq, ndq);
+        acc[j] += share * (kd.d / kd.v) * gl;
+      }
+    }
+  }
+  wave_sums(acc);
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < 9; j++) grc[9 * r + j] = acc[j];
   }
 }

@@ -9,9 +9,13 @@ tangent through dt or h, and none of h out. The tangents of x, T, P and J enter 
 CudaSolver.launch's, bit for bit (the same kernels' arithmetic in the same order). A cell the guard fails is
 recomputed by the recovery without a tangent: its tangents are NaN, its flag 1 or 2 as in launch.
 
-The adjoint (vjp) of the same map is assembled from the tangent's columns, one tangent call per input coordinate.
+Beside them, both contracts take per-cell log rate multipliers lm [B, R] (a reaction's two coefficients times exp(lm)
+after the collision limit, a photolysis branch's kf = J exp(lm)) and another rate table rc [R, W] for the call, with
+their tangents dlm and drc (the nine Arrhenius columns of rc).
 
-Not built (refused by name): the rate multipliers lm and the rate table rc with their derivatives.
+The adjoint (vjp) of the same map is assembled from the tangent's columns, one tangent call per input coordinate; the
+rate table's cotangent follows from the multipliers' (a multiplier and the table's law enter a reaction's
+coefficients alike) through one reduction kernel.
 """
 
 from __future__ import annotations
@@ -28,6 +32,7 @@ from .kincuda import CudaSolver
 
 __all__ = ["ADSolver", "Cotangents", "JvpResult", "VjpResult"]
 DMAX = 64  # directions of one jvp call (the reference's bound on the leading axis)
+ARRHENIUS_COLUMNS = 9  # ln A, B, C of k0, of the second k0 and of k_inf: the differentiated columns of rc
 
 
 class JvpResult(NamedTuple):
@@ -44,8 +49,8 @@ class JvpResult(NamedTuple):
 
 class Cotangents(NamedTuple):
     """The cotangents of ADSolver.vjp's inputs (the reference's Cotangents), float64, each per unit of its input: x
-    [B, S]; T, P, dt, h [B]; lm [B, R] and rc [R, 9] (None: not built here); J [B, branches] (None without
-    photolysis)."""
+    [B, S]; T, P, dt, h [B]; lm [B, R] and rc [R, 9] (None when vjp is asked not to build them); J [B, branches]
+    (None without photolysis)."""
 
     x: torch.Tensor
     T: torch.Tensor
@@ -91,6 +96,35 @@ class ADSolver(CudaSolver):
                 raise ValueError(f"one cell's tangent kernel needs {self.lds_jvp} B of LDS; the device has "
                                  f"{info['lds_per_block']}")  # fmt: skip
             self.max_blocks_jvp = info["compute_units"] * max(1, _hip.occupancy(self._jvp, WAVE, self.lds_jvp))
+            self._recover_lm = self._module.function("mxc_recover_lm")
+            self._rc_cotangent = self._module.function("mxc_rc_cotangent")
+        # the rate table's width as the reference gives it: the nine Arrhenius columns, and the Troe parameters and
+        # the efficiency when a reaction has them
+        self.rcw = self.tab.rc.shape[1] if self.tab.has_troe or self.tab.has_eff else ARRHENIUS_COLUMNS
+
+    @property
+    def rc0(self) -> torch.Tensor:
+        """The network's rate table [R, W] (the reference's ADSolver.rc0), which a call uses unless given another: per
+        reaction ln A, B, C of k0, of the second k0 and of k_inf, then (W = 14) Troe a, T3, T1, T2 and the efficiency;
+        photolysis rows are 0."""
+        self._require_open()
+        return self._tables["rc"][:, : self.rcw]
+
+    def _rates(self, B: int, lm, rc):
+        """Check lm [B, R] and rc [R, W]; return lm contiguous (or None), the Net struct of the call (the solver's, or a
+        copy that points at rc widened to the kernels' 14 columns) and the widened table, which the caller keeps until
+        the call's kernels have been launched."""
+        if lm is not None:
+            check_tensor("lm", lm, (B, self.nr), self.dev)
+            lm = lm.contiguous()
+        if rc is None:
+            return lm, self._net, None
+        check_tensor("rc", rc, (self.nr, self.rcw), self.dev)
+        full = rc if self.rcw == self.tab.rc.shape[1] else torch.cat([rc, self._tables["rc"][:, self.rcw:]], 1)
+        full = full.contiguous()
+        net = self._net.clone()
+        net[_kernels.NET_FIELDS.index("rc")] = full.data_ptr()
+        return lm, net, full
 
     def jvp(self, x, T, P, dt, h, dx, dT, dP, ddt, dh, *, lm=None, dlm=None, rc=None, drc=None, adapt: bool = True,
             rtol: float = DEFAULT_RTOL, atol: float = ATOL, max_steps: int = DEFAULT_MAX_STEPS, J=None,
@@ -102,12 +136,14 @@ class ADSolver(CudaSolver):
         differentiates the step control too: every try's tangent, the error's and the step factor's, so ddt and dh
         enter and the result's h has a tangent; adapt=False holds the accepted steps fixed (ddt and dh do not enter,
         the result's dh is zero). The result's x, h, done and steps are CudaSolver.launch's; dx, dh the tangents of x
-        and h (NaN for a cell the guard lists). Refusals and counts as launch's."""
+        and h (NaN for a cell the guard lists). Refusals and counts as launch's.
+
+        lm [B, R]: per-cell log rate multipliers (a reaction's two coefficients times exp(lm) after the collision
+        limit, a photolysis branch's kf = J exp(lm)); a non-finite one makes its cell bad input. rc [R, W]: the rate
+        table the call uses instead of rc0. dlm [B, R] and drc [R, 9] (the nine Arrhenius columns), or with the
+        leading axis: their tangents."""
         self.flag = None
         self._require_open()
-        for name, a in (("lm", lm), ("dlm", dlm), ("rc", rc), ("drc", drc)):
-            if a is not None:
-                raise NotImplementedError(f"{name}: rate multipliers and the rate table are not differentiated here")
         B = check_inputs(self.dev, self.ns, x, T, P, dt, h, rtol, atol, max_steps)
         if dx.dim() not in (2, 3):
             raise ValueError(f"dx: [B, S] or [D, B, S] expected, got {list(dx.shape)}")
@@ -120,17 +156,21 @@ class ADSolver(CudaSolver):
             raise ValueError(f"dJ: none or an empty {[*lead, B, 0]} one expected, since this solver has no photolysis; "
                              f"got {list(dJ.shape)}")  # fmt: skip
         shapes = {"dx": (*lead, B, self.ns), "dT": (*lead, B), "dP": (*lead, B), "ddt": (*lead, B), "dh": (*lead, B),
-                  "dJ": (*lead, B, self.nj)}  # fmt: skip
-        for name, a in (("dx", dx), ("dT", dT), ("dP", dP), ("ddt", ddt), ("dh", dh), ("dJ", dJ)):
+                  "dJ": (*lead, B, self.nj), "dlm": (*lead, B, self.nr), "drc": (*lead, self.nr, ARRHENIUS_COLUMNS)}
+        tangents = (("dx", dx), ("dT", dT), ("dP", dP), ("ddt", ddt), ("dh", dh), ("dJ", dJ), ("dlm", dlm),
+                    ("drc", drc))  # fmt: skip
+        for name, a in tangents:
             if a is not None:
                 check_tensor(name, a, shapes[name], self.dev)
         with torch.cuda.device(self.dev):
             x, T, P, dt, h = (a.contiguous() for a in (x, T, P, dt, h))
+            lm, net, table = self._rates(B, lm, rc)
             jx = check_photolysis(J, B, self.nj, self.dev)
             dJ = dJ if self.nj else None
             tx = dx if not one else dx[None]
             tT, tP, tdt, th = (None if a is None else (a if not one else a[None]) for a in (dT, dP, ddt, dh))
             tJ = None if dJ is None else (dJ if not one else dJ[None])
+            tlm, trc = (None if a is None else (a if not one else a[None]) for a in (dlm, drc))
             xd = torch.empty(D, B, self.ns, dtype=torch.float64, device=self.dev)
             hd = torch.empty(D, B, dtype=torch.float64, device=self.dev)
             result = None
@@ -140,7 +180,9 @@ class ADSolver(CudaSolver):
 
             for d in range(D):  # each direction repeats the call, with the same bits
                 result = self._jvp_once(x, T, P, dt, h, jx, rtol, atol, max_steps, B, tx[d].contiguous(), at(tT, d),
-                                        at(tP, d), at(tJ, d), xd[d], adapt, at(tdt, d), at(th, d), hd[d])  # fmt: skip
+                                        at(tP, d), at(tJ, d), xd[d], adapt, at(tdt, d), at(th, d), hd[d], lm,
+                                        at(tlm, d), at(trc, d), net)  # fmt: skip
+            del table  # kept to here: the launches above read it
             xo, ho, done, nok, redo = result
             self.total += self.count
             self._refuse(x, T, P, dt, jx, self._status)
@@ -148,7 +190,7 @@ class ADSolver(CudaSolver):
         return JvpResult(xo, ho, done.bool(), nok, xd[0] if one else xd, hd[0] if one else hd)
 
     def _jvp_once(self, x, T, P, dt, h, jx, rtol, atol, max_steps, B, dx, dT, dP, dJ, xd, adapt=False, ddt=None,
-                  dh=None, hd=None):  # fmt: skip
+                  dh=None, hd=None, lm=None, dlm=None, drc=None, net=None):  # fmt: skip
         xo, ho = torch.empty_like(x), h.clone()
         done, redo, lst, status = (torch.empty(B, dtype=torch.int32, device=self.dev) for _ in range(4))
         nok = torch.empty(B, dtype=torch.float64, device=self.dev)
@@ -156,37 +198,40 @@ class ADSolver(CudaSolver):
         self.count.zero_()
         p = _hip.ptr
         stream = torch.cuda.current_stream(self.dev).cuda_stream
-        args = [p(self._net), p(x), p(T), p(P), p(dt), p(ho), p(xo), p(done), p(nok), p(status), p(redo), p(lst),
+        net = self._net if net is None else net
+        args = [p(net), p(x), p(T), p(P), p(dt), p(ho), p(xo), p(done), p(nok), p(status), p(redo), p(lst),
                 p(self.count), p(self.tries), p(jx), ctypes.c_int(B), ctypes.c_double(rtol), ctypes.c_double(atol),
                 ctypes.c_int(max_steps), p(dx), p(dT), p(dP), p(dJ), p(xd), ctypes.c_int(int(bool(adapt))), p(ddt),
-                p(dh), p(hd)]  # fmt: skip
+                p(dh), p(hd), p(lm), p(dlm), p(drc)]  # fmt: skip
         if B:
             _hip.launch(self._jvp, max(1, min(B, self.max_blocks_jvp)), WAVE, self.lds_jvp, stream, args)
         # the cells the guard failed: the recovery's result, without a tangent (NaN from the tangent kernel)
-        rec_args = [p(self._net), p(x), p(T), p(P), p(dt), p(h), p(xo), p(ho), p(done), p(nok), p(redo), p(lst),
+        rec_args = [p(net), p(x), p(T), p(P), p(dt), p(h), p(xo), p(ho), p(done), p(nok), p(redo), p(lst),
                     p(self.count), p(self.tries), p(jx), ctypes.c_double(rtol), ctypes.c_double(atol),
                     ctypes.c_int(REC_STEPS * max_steps), p(self._rscratch)]  # fmt: skip
-        _hip.launch(self._recover, max(1, min(B, self.rec_blocks)), WAVE * _kernels.REC_WAVES, self.lds_rec, stream,
-                    rec_args)  # fmt: skip
+        recover = self._recover if lm is None else self._recover_lm
+        _hip.launch(recover, max(1, min(B, self.rec_blocks)), WAVE * _kernels.REC_WAVES, self.lds_rec, stream,
+                    rec_args + ([] if lm is None else [p(lm)]))  # fmt: skip
         self._status = status
         return xo, ho, done, nok, redo
 
     def vjp(self, x, T, P, dt, h, gx, *, gh=None, lm=None, rc=None, adapt: bool = True, rtol: float = DEFAULT_RTOL,
-            atol: float = ATOL, max_steps: int = DEFAULT_MAX_STEPS, J=None) -> VjpResult:  # fmt: skip
+            atol: float = ATOL, max_steps: int = DEFAULT_MAX_STEPS, J=None, rates: bool = True) -> VjpResult:  # fmt: skip
         """The call and its inputs' cotangents for the output cotangents gx [B, S] and gh [B] (optional), under
         either contract as jvp (the reference's ADSolver.vjp, maxichem/kincuda_ad.py:457-556). Everything after gx is
-        keyword only.
+        keyword only; lm [B, R] and rc [R, W] as jvp.
 
         Built from the tangent by columns, not by a reverse pass: one tangent call per input coordinate (each of x's
-        S entries, T, P, each photolysis branch, and under adapt=True dt and h), every cell along the same unit
-        direction at once, and each cell's cotangent gx . dx + gh dh of its column. It costs S + 2 + branches (+ 2)
-        tangent calls, each a call with its tangent. Under the frozen contract the cotangents of dt and h are zero.
-        Those of lm and rc are not built (None). A cell the guard lists has NaN cotangents, its flag as launch's."""
+        S entries, T, P, each photolysis branch, under adapt=True dt and h, and with rates each thermal reaction's
+        multiplier), every cell along the same unit direction at once, and each cell's cotangent gx . dx + gh dh of
+        its column. A photolysis branch's multiplier cotangent is J times J's (kf = J exp(lm)); the rate table's is
+        reduced from the multipliers' by mxc_rc_cotangent (a multiplier and the law enter a reaction's coefficients
+        alike, the law's share 1 - wA of the collision limit). It costs S + 2 + branches (+ 2) tangent calls, and the
+        thermal reactions' count more with rates (rates=False leaves the cotangents of lm and rc None). Under the
+        frozen contract the cotangents of dt and h are zero. A cell the guard lists has NaN cotangents, its flag as
+        launch's, and is left out of the rate table's sum (fallback["recomputed"] counts them)."""
         self.flag = None
         self._require_open()
-        for name, a in (("lm", lm), ("rc", rc)):
-            if a is not None:
-                raise NotImplementedError(f"{name}: rate multipliers and the rate table are not differentiated here")
         B = check_inputs(self.dev, self.ns, x, T, P, dt, h, rtol, atol, max_steps)
         check_tensor("gx", gx, (B, self.ns), self.dev)
         if gh is not None:
@@ -194,22 +239,26 @@ class ADSolver(CudaSolver):
         with torch.cuda.device(self.dev):
             x, T, P, dt, h, gx = (a.contiguous() for a in (x, T, P, dt, h, gx))
             gh = None if gh is None else gh.contiguous()
+            lm, net, table = self._rates(B, lm, rc)
             jx = check_photolysis(J, B, self.nj, self.dev)
             zero = torch.zeros(B, dtype=torch.float64, device=self.dev)
             col = torch.empty(B, self.ns, dtype=torch.float64, device=self.dev)
             colh = torch.empty(B, dtype=torch.float64, device=self.dev)
             g = Cotangents(x=torch.empty_like(x), T=torch.empty_like(T), P=torch.empty_like(T), dt=zero.clone(),
-                           h=zero.clone(), lm=None, rc=None,
+                           h=zero.clone(),
+                           lm=torch.empty(B, self.nr, dtype=torch.float64, device=self.dev) if rates else None,
+                           rc=torch.empty(self.nr, ARRHENIUS_COLUMNS, dtype=torch.float64, device=self.dev)
+                           if rates else None,
                            J=torch.empty(B, self.nj, dtype=torch.float64, device=self.dev) if self.nj else None)  # fmt: skip
             unit = torch.zeros(B, self.ns, dtype=torch.float64, device=self.dev)
             unit_j = torch.zeros(B, self.nj, dtype=torch.float64, device=self.dev) if self.nj else None
             ones = torch.ones(B, dtype=torch.float64, device=self.dev)
             result = None
 
-            def column(dx, dT=None, dP=None, dJ=None, ddt=None, dh=None):
+            def column(dx, dT=None, dP=None, dJ=None, ddt=None, dh=None, dlm=None):
                 nonlocal result
                 result = self._jvp_once(x, T, P, dt, h, jx, rtol, atol, max_steps, B, dx, dT, dP, dJ, col, adapt, ddt,
-                                        dh, colh)  # fmt: skip
+                                        dh, colh, lm, dlm, None, net)  # fmt: skip
                 out = (gx * col).sum(1)
                 return out if gh is None or not adapt else out + gh * colh
 
@@ -227,7 +276,23 @@ class ADSolver(CudaSolver):
             if adapt:  # dt and h enter under the adaptive contract
                 g.dt.copy_(column(unit, ddt=ones))
                 g.h.copy_(column(unit, dh=ones))
+            if rates:
+                nrt = self.tab.nrt
+                unit_r = torch.zeros(B, self.nr, dtype=torch.float64, device=self.dev)
+                for r in range(nrt):  # the thermal reactions' multipliers
+                    unit_r.zero_()
+                    unit_r[:, r] = 1.0
+                    g.lm[:, r] = column(unit, dlm=unit_r)
+                if self.nj:  # a branch's kf = J exp(lm): d/dlm = J d/dJ
+                    g.lm[:, nrt:] = jx * g.J
             xo, ho, done, nok, redo = result
+            if rates:
+                skip = redo.to(torch.int32)  # the recomputed cells have no tangent: left out of the table's sum
+                stream = torch.cuda.current_stream(self.dev).cuda_stream
+                _hip.launch(self._rc_cotangent, self.nr, WAVE, 0, stream,
+                            [_hip.ptr(net), _hip.ptr(T), _hip.ptr(P), _hip.ptr(g.lm), _hip.ptr(skip), ctypes.c_int(B),
+                             _hip.ptr(g.rc)])  # fmt: skip
+            del table
             self.total += self.count
             self._refuse(x, T, P, dt, jx, self._status)
         self.flag = redo

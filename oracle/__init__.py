@@ -47,6 +47,7 @@ def lib() -> ctypes.CDLL:
             build()
         _lib = ctypes.CDLL(str(LIB))
         _lib.oracle_call.restype = ctypes.c_int
+        _lib.oracle_call_lm.restype = ctypes.c_int
         _lib.oracle_constant.restype = ctypes.c_double
         _lib.oracle_constant.argtypes = [ctypes.c_char_p]
         _lib.oracle_step_factor.restype = ctypes.c_double
@@ -78,8 +79,9 @@ class Oracle:
             _p(atoms, _d), float(drift),
         )  # fmt: skip
 
-    def call(self, x, T, P, dt, h, *, rtol=1e-3, atol=1e-25, max_steps=4000, J=None, threads=0):
-        """One call of the solver on the CPU: dict of x, h, done, steps, status, flag, tries and count [12]."""
+    def call(self, x, T, P, dt, h, *, rtol=1e-3, atol=1e-25, max_steps=4000, J=None, threads=0, lm=None):
+        """One call of the solver on the CPU: dict of x, h, done, steps, status, flag, tries and count [12]. lm: the
+        cells' log rate multipliers [B, nr] (the tangent kernels' primal), or None."""
         x = np.ascontiguousarray(x, np.float64)
         B = x.shape[0]
         T, P, dt, h = (np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (B,))) for a in (T, P, dt, h))
@@ -93,8 +95,13 @@ class Oracle:
             "status": np.empty(B, np.int32), "flag": np.empty(B, np.int32), "tries": np.empty(B),
             "count": np.zeros(OC_SLOTS, np.int64),
         }  # fmt: skip
-        lib().oracle_call(
-            ctypes.byref(self.net), ctypes.c_int(B), _p(x, _d), _p(T, _d), _p(P, _d), _p(dt, _d), _p(h, _d), Jp,
+        lmp = None
+        if lm is not None:
+            lm = np.ascontiguousarray(lm, np.float64)
+            assert lm.shape == (B, self.tab.nr)
+            lmp = _p(lm, _d)
+        lib().oracle_call_lm(
+            ctypes.byref(self.net), ctypes.c_int(B), _p(x, _d), _p(T, _d), _p(P, _d), _p(dt, _d), _p(h, _d), Jp, lmp,
             ctypes.c_double(rtol), ctypes.c_double(atol), ctypes.c_int(max_steps), _p(out["x"], _d),
             _p(out["h"], _d), _p(out["done"], _i), _p(out["steps"], _d), _p(out["status"], _i), _p(out["flag"], _i),
             _p(out["tries"], _d), out["count"].ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),

@@ -139,7 +139,10 @@ static int n_reverse(int fl) { return (fl >> FL_NR_SHIFT) & 3; }
 
 /* kf, kr of every reaction (common.cuh:208-242): k from the law; three-body both ways times nd; the reverse from
  * exp(dG/RT) (k_B T / P0)^dnu; then the collision limit (common.cuh:171-180). Photolysis: kf = J, kr = 0. */
-void oracle_rates(const oracle_net* net, double T, double P, const double* J, double* kf, double* kr) {
+/* lm: the cell's log rate multipliers [nr] or NULL: both coefficients of a reaction times exp(lm) after the
+ * collision limit, a photolysis branch's kf = J exp(lm) (ad_common.cuh:136-168) */
+static void rates_lm(const oracle_net* net, double T, double P, const double* J, const double* lm, double* kf,
+                     double* kr) {
   const int ns = net->ns;
   const double nd = P / (KB * T), lT = log(T), iT = 1.0 / T, lKT = log(KB * T / P0);
   double* g = (double*)malloc(sizeof(double) * ns);
@@ -149,6 +152,7 @@ void oracle_rates(const oracle_net* net, double T, double P, const double* J, do
     if (fl & FL_PHOTO) {
       kf[r] = J ? J[r - net->nrt] : 0.0;
       kr[r] = 0.0;
+      if (lm) kf[r] *= exp(lm[r]);
       continue;
     }
     const int* x = net->rx + 6 * r;
@@ -165,10 +169,19 @@ void oracle_rates(const oracle_net* net, double T, double P, const double* J, do
       f *= c;
       b *= c;
     }
+    if (lm) {
+      const double e = exp(lm[r]);
+      f *= e;
+      b *= e;
+    }
     kf[r] = f;
     kr[r] = b;
   }
   free(g);
+}
+
+void oracle_rates(const oracle_net* net, double T, double P, const double* J, double* kf, double* kr) {
+  rates_lm(net, T, P, J, NULL, kf, kr);
 }
 
 /* ---- the cell's work space ---- */
@@ -594,8 +607,8 @@ int oracle_check_cell(const oracle_net* net, const double* x0, double* xout, int
 /* ---- one cell: screen, call, guard, recovery ---- */
 
 static void one_cell(cell_t* c, const int* inv, int b, const double* x0, double T, double P, double dt, double hin,
-                     const double* J, double rtol, double atol, int max_steps, double* xout, double* hout, int* done,
-                     double* nok, int* status, int* flag, double* tries, long long* count) {
+                     const double* J, const double* lm, double rtol, double atol, int max_steps, double* xout,
+                     double* hout, int* done, double* nok, int* status, int* flag, double* tries, long long* count) {
   const oracle_net* net = c->net;
   const int ns = c->ns, nj = net->nj;
   const double nd = P / (KB * T);
@@ -608,11 +621,12 @@ static void one_cell(cell_t* c, const int* inv, int b, const double* x0, double 
     pm = fmax(pm, c->N[i]);
   }
   for (int q = 0; q < nj && J; q++) p += J[q] >= 0.0 ? 0.0 * J[q] : NAN;
+  for (int r = 0; r < net->nr && lm; r++) p += 0.0 * lm[r]; /* a non-finite multiplier: bad input (ad_common.cuh:38-43) */
   const int fine = fabs(p - 1.0) <= X_SUM_TOL && pm > 0.0 && isfinite(T) && isfinite(nd) && nd > 0.0 &&
                    isfinite(dt) && dt >= 0.0;
   const int st = T < T_MIN ? REFUSED_COLD : (T > T_MAX ? REFUSED_HOT : (fine ? 0 : REFUSED_BAD_INPUT));
   *status = st;
-  if (st == 0) oracle_rates(net, T, P, J, c->kf, c->kr);
+  if (st == 0) rates_lm(net, T, P, J, lm, c->kf, c->kr);
   const double h0 = isfinite(hin) ? hin : dt;
   if (!isfinite(hin)) count[OC_NONFINITE_H]++;
   step_t s;
@@ -659,10 +673,11 @@ static void one_cell(cell_t* c, const int* inv, int b, const double* x0, double 
   }
 }
 
-int oracle_call(const oracle_net* net, int B, const double* x0, const double* T, const double* P, const double* dt,
-                const double* hin, const double* J, double rtol, double atol, int max_steps, double* xout,
-                double* hout, int* done, double* nok, int* status, int* flag, double* tries, long long* count,
-                int threads) {
+/* oracle_call with per-cell log rate multipliers lm [B][nr] (NULL: none), the tangent kernels' primal */
+int oracle_call_lm(const oracle_net* net, int B, const double* x0, const double* T, const double* P, const double* dt,
+                   const double* hin, const double* J, const double* lm, double rtol, double atol, int max_steps,
+                   double* xout, double* hout, int* done, double* nok, int* status, int* flag, double* tries,
+                   long long* count, int threads) {
   const int ns = net->ns, nj = net->nj;
   int* inv = (int*)malloc(sizeof(int) * ns);
   for (int k = 0; k < ns; k++) inv[net->perm[k]] = k;
@@ -681,8 +696,9 @@ int oracle_call(const oracle_net* net, int B, const double* x0, const double* T,
 #pragma omp for schedule(dynamic, 1)
     for (int b = 0; b < B; b++) {
       tries[b] = 0.0;
-      one_cell(&c, inv, b, x0 + (size_t)b * ns, T[b], P[b], dt[b], hin[b], J ? J + (size_t)b * nj : NULL, rtol, atol,
-               max_steps, xout + (size_t)b * ns, hout + b, done + b, nok + b, status + b, flag + b, tries + b, mine);
+      one_cell(&c, inv, b, x0 + (size_t)b * ns, T[b], P[b], dt[b], hin[b], J ? J + (size_t)b * nj : NULL,
+               lm ? lm + (size_t)b * net->nr : NULL, rtol, atol, max_steps, xout + (size_t)b * ns, hout + b, done + b,
+               nok + b, status + b, flag + b, tries + b, mine);
     }
 #pragma omp critical
     for (int i = 0; i < OC_SLOTS; i++) total[i] += mine[i];
@@ -692,6 +708,14 @@ int oracle_call(const oracle_net* net, int B, const double* x0, const double* T,
   free(col);
   free(inv);
   return 0;
+}
+
+int oracle_call(const oracle_net* net, int B, const double* x0, const double* T, const double* P, const double* dt,
+                const double* hin, const double* J, double rtol, double atol, int max_steps, double* xout,
+                double* hout, int* done, double* nok, int* status, int* flag, double* tries, long long* count,
+                int threads) {
+  return oracle_call_lm(net, B, x0, T, P, dt, hin, J, NULL, rtol, atol, max_steps, xout, hout, done, nok, status,
+                        flag, tries, count, threads);
 }
 
 /* The step-size law alone (step_factor above), for the tests that pin it against known answers. */

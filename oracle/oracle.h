@@ -37,6 +37,14 @@ int oracle_call(const oracle_net* net, int B, const double* x0, const double* T,
                 double* hout, int* done, double* nok, int* status, int* flag, double* tries, long long* count,
                 int threads);
 
+/* oracle_call with the cells' log rate multipliers lm [B][nr] (NULL: none): both coefficients of a reaction times
+ * exp(lm) after the collision limit, a photolysis branch's kf = J exp(lm); a non-finite multiplier makes its cell bad
+ * input. */
+int oracle_call_lm(const oracle_net* net, int B, const double* x0, const double* T, const double* P, const double* dt,
+                   const double* hin, const double* J, const double* lm, double rtol, double atol, int max_steps,
+                   double* xout, double* hout, int* done, double* nok, int* status, int* flag, double* tries,
+                   long long* count, int threads);
+
 /* Rate coefficients of one cell (T, P, J) before any call: kf, kr [nr] (network reaction order). */
 void oracle_rates(const oracle_net* net, double T, double P, const double* J, double* kf, double* kr);
 

@@ -10,8 +10,16 @@ against the call, tests/test_kernel_copies.py).
 - GPU: jvp's x, h, done and steps are CudaSolver.launch's bit for bit; its tangent along random directions of x, T,
   P and J matches the central differences to 1e-5 of each cell's largest tangent entry; several directions in one
   call give each direction's tangent; vjp (built from the tangent's columns) is its transpose by the dot-product
-  test; the refusals of what is not built.
+  test; inputs of the wrong shape are refused.
+- The rate multipliers lm and the rate table rc: on the CPU, doubling every coefficient (lm = ln 2) is the same call
+  over twice the time with twice the first step, bit for bit, on the oracle; on the GPU the same identity holds for
+  the tangent kernel's primal, the tangent along lm = 1 is the tangent along (dt, h) = (dt, h), the tangents along
+  lm and the Arrhenius columns of rc match the whole call's differences (the oracle with lm and another table), and
+  vjp's cotangents of lm (per cell) and rc (summed over the cells) pass the dot-product test.
 """
+
+import dataclasses
+import math
 
 import numpy as np
 import oracle
@@ -122,7 +130,7 @@ def test_jvp_repeats_the_call_and_matches_the_frozen_differences(gpu):
 
 
 @pytest.mark.gpu
-def test_directions_share_one_call_and_unbuilt_parts_are_refused(gpu):
+def test_directions_share_one_call_and_bad_shapes_are_refused(gpu):
     from maxichem_amd.kincuda_ad import ADSolver
 
     net, _, inp, d = case(seed=7)
@@ -138,9 +146,12 @@ def test_directions_share_one_call_and_unbuilt_parts_are_refused(gpu):
         assert torch.equal(both.dx[0], one[0]) and torch.equal(both.dx[1], one[1])
         ok = torch.isfinite(one[0]).all(1)
         assert torch.allclose(one[1][ok], -2.0 * one[0][ok], rtol=1e-9, atol=0.0)  # linear in the direction
-        with pytest.raises(NotImplementedError, match="lm"):
+        with pytest.raises(ValueError, match="lm"):
             ad.jvp(*args, t(d["dx"]), zero, zero, zero, zero, adapt=False, J=t(inp["J"]),
-                   lm=torch.zeros(CELLS, ad.nr, dtype=torch.float64, device="cuda"))  # fmt: skip
+                   lm=torch.zeros(CELLS, ad.nr - 1, dtype=torch.float64, device="cuda"))  # fmt: skip
+        with pytest.raises(ValueError, match="drc"):
+            ad.jvp(*args, t(d["dx"]), zero, zero, zero, zero, adapt=False, J=t(inp["J"]),
+                   drc=torch.zeros(ad.nr, ad.rcw, dtype=torch.float64, device="cuda"))  # fmt: skip
         with pytest.raises(ValueError, match="dx"):
             ad.jvp(*args, t(d["dx"][:, :5]), zero, zero, zero, zero, adapt=False, J=t(inp["J"]))
 
@@ -160,12 +171,9 @@ def test_vjp_is_the_transpose_of_jvp(gpu, adapt):
     with ADSolver(net, "cuda", photo=True) as ad:
         r = ad.jvp(*args, t(d["dx"]), t(d["dT"]), t(d["dP"]), t(d["ddt"]), t(d["dh"]), adapt=adapt, J=t(inp["J"]),
                    dJ=t(d["dJ"]))  # fmt: skip
-        v = ad.vjp(*args, t(gx), gh=t(gh), adapt=adapt, J=t(inp["J"]))
+        v = ad.vjp(*args, t(gx), gh=t(gh), adapt=adapt, J=t(inp["J"]), rates=False)
         for name in ("x", "h", "done", "steps"):
             assert torch.equal(getattr(v, name), getattr(r, name)), name
-        with pytest.raises(NotImplementedError, match="lm"):
-            ad.vjp(*args, t(gx), adapt=adapt, J=t(inp["J"]), lm=torch.zeros(CELLS, ad.nr, dtype=torch.float64,
-                                                                             device="cuda"))  # fmt: skip
     g = v.cotangents
     assert g.lm is None and g.rc is None
     if not adapt:
@@ -205,10 +213,15 @@ def adaptive_differences(orc, inp, d):
         return orc.call(inp["x"] + e * d["dx"], inp["T"] + e * d["dT"], inp["P"] + e * d["dP"], inp["dt"] + e * d["ddt"],
                         inp["h"] + e * d["dh"], J=inp["J"] + e * d["dJ"])  # fmt: skip
 
+    return call_differences(call)
+
+
+def call_differences(call, steps=ADAPT_EPS):
+    """adaptive_differences of call(e), the oracle's call at e along a direction, at the two steps."""
     base = call(0.0)
     keep = base["flag"] == 0
     fd = []
-    for e in ADAPT_EPS:
+    for e in steps:
         up, down = call(e), call(-e)
         for r in (up, down):
             keep &= (r["tries"] == base["tries"]) & (r["steps"] == base["steps"]) & (r["flag"] == 0)
@@ -255,3 +268,207 @@ def test_adaptive_tangent_matches_the_differences_of_the_call(gpu):
           f"{err.max():.2e}, of h {errh.max():.2e}")  # fmt: skip
     assert use.sum() >= CELLS // 3 and use_h.sum() >= CELLS // 3
     assert err.max() <= ADAPT_TOLERANCE and errh.max() <= ADAPT_TOLERANCE
+
+
+# ---- the rate multipliers lm and the rate table rc ----
+
+LN2 = math.log(2.0)  # exp(LN2) is 2.0 exactly in double (asserted below): lm = LN2 doubles every coefficient exactly
+ARRHENIUS = 9  # rc's differentiated columns: ln A, B, C of k0, of the second k0 and of k_inf
+
+
+def test_the_oracles_doubled_coefficients_are_twice_the_time():
+    # dN/dt = 2 F(N) over dt from h is dN/ds = F(N) over 2 dt from 2 h: each Rodas3 try's W = I / (gamma h) - 2 Jn
+    # is 2 (I / (gamma 2h) - Jn) and its right-hand sides double, so every stage is the step 2h's, and doubling is
+    # exact in floating point. The two calls agree bit for bit, h twice, the same tries. lm on one direction only,
+    # before the collision limit (every cell here has reactions at the limit) or not on the photolysis branches
+    # breaks it.
+    assert math.exp(LN2) == 2.0
+    net, orc, inp, _ = adaptive_case()
+    lm = np.full((CELLS, orc.tab.nr), LN2)
+    a = orc.call(inp["x"], inp["T"], inp["P"], inp["dt"], inp["h"], J=inp["J"], lm=lm)
+    b = orc.call(inp["x"], inp["T"], inp["P"], 2 * inp["dt"], 2 * inp["h"], J=inp["J"])
+    assert np.array_equal(a["x"], b["x"]) and np.array_equal(2 * a["h"], b["h"])
+    for k in ("steps", "tries", "flag", "done"):
+        assert np.array_equal(a[k], b[k]), k
+    none = orc.call(inp["x"], inp["T"], inp["P"], inp["dt"], inp["h"], J=inp["J"])
+    zero = orc.call(inp["x"], inp["T"], inp["P"], inp["dt"], inp["h"], J=inp["J"], lm=np.zeros_like(lm))
+    assert np.array_equal(zero["x"], none["x"]) and np.array_equal(zero["h"], none["h"])
+    bad = np.zeros_like(lm)
+    bad[3, 7] = np.inf  # a non-finite multiplier: its cell is bad input, the others are untouched
+    out = orc.call(inp["x"], inp["T"], inp["P"], inp["dt"], inp["h"], J=inp["J"], lm=bad)
+    assert out["status"][3] != 0 and not np.delete(out["status"], 3).any()
+    assert np.array_equal(np.delete(out["x"], 3, 0), np.delete(none["x"], 3, 0))
+
+
+# the rate directions' differences: on cells that have not settled the call takes up to 270 tries, whose rounding
+# makes the differences at step 1e-6 disagree by up to 1e-3; at these steps they agree to RATE_AGREE on most cells
+RATE_EPS = (1e-5, 1e-4)
+RATE_AGREE = 1e-4
+RATE_TOLERANCE = 3e-4  # of a cell's largest entry: the differences at 1e-5 are within about RATE_AGREE of the limit
+MOVES = 1e-12  # a cell whose x tangent stays below this does not move along the direction (rounding only)
+
+
+def cold_case(seed: int = 5):
+    """NCHO cells from the inputs' own states (not settled), dt 300 s, a quarter lit, so that a call moves them:
+    a settled cell's equilibrium depends neither on a multiplier nor on an A that scales both directions alike."""
+    net = standin.load("NCHO")
+    orc = oracle.Oracle(_tables.build(net, True))
+    raw = workloads.config1(net.species, len(net.photo), cells=CELLS, seed=seed)
+    return net, orc, {k: raw[k] for k in ("x", "T", "P", "dt", "h", "J")}
+
+
+def rate_directions(orc, seed: int):
+    """A base lm (up to 0.1 per reaction and cell) and directions: of lm (up to 0.1) and of rc's Arrhenius columns on
+    the thermal rows (ln A by up to 0.05, B by up to 0.01, C by up to 0.5 % of |C|)."""
+    rng = np.random.default_rng(seed)
+    tab = orc.tab
+    lm = 0.1 * rng.uniform(-1, 1, (CELLS, tab.nr))
+    dlm = 0.1 * rng.uniform(-1, 1, (CELLS, tab.nr))
+    drc = np.zeros((tab.nr, ARRHENIUS))
+    th = slice(0, tab.nrt)
+    for j in range(ARRHENIUS):
+        u = rng.uniform(-1, 1, tab.nrt)
+        drc[th, j] = (0.05, 0.01, 0.005 * np.abs(tab.rc[th, j]))[j % 3] * u
+    return lm, dlm, drc
+
+
+def with_table(orc, drc, e: float = 1.0):
+    """An oracle over the same network with rc's Arrhenius columns moved by e drc."""
+    rc = orc.tab.rc.copy()
+    rc[:, :ARRHENIUS] += e * drc
+    return oracle.Oracle(dataclasses.replace(orc.tab, rc=rc))
+
+
+def zeros_like_inputs(inp):
+    return t(np.zeros_like(inp["x"])), t(np.zeros(CELLS))
+
+
+@pytest.mark.gpu
+def test_multipliers_and_table_in_the_call(gpu):
+    # the tangent kernel's primal with lm = ln 2 is the plain call over twice the time from twice h, bit for bit (the
+    # identity above); with random multipliers and another rate table it meets the plain call's parity bar against
+    # the oracle's call with the same
+    from test_parity import compare
+
+    from maxichem_amd.kincuda_ad import ADSolver
+
+    net, orc, inp, _ = adaptive_case()
+    lm, _, drc = rate_directions(orc, 17)
+    args = [t(inp[k]) for k in ("x", "T", "P", "dt", "h")]
+    zx, z = zeros_like_inputs(inp)
+    with plain_solver(net) as plain:
+        want = plain.launch(args[0], args[1], args[2], 2 * args[3], 2 * args[4], J=t(inp["J"]))
+    with ADSolver(net, "cuda", photo=True) as ad:
+        assert ad.rc0.shape == (ad.nr, ad.rcw)
+        r = ad.jvp(*args, zx, z, z, z, z, lm=t(np.full((CELLS, ad.nr), LN2)), J=t(inp["J"]))
+        assert torch.equal(r.x, want.x) and torch.equal(2 * r.h, want.h) and torch.equal(r.steps, want.steps)
+        rc = ad.rc0.clone()
+        rc[:, :ARRHENIUS] += t(drc)
+        r = ad.jvp(*args, zx, z, z, z, z, lm=t(lm), rc=rc, J=t(inp["J"]))
+        got = {**{k: v.cpu().numpy() for k, v in r._asdict().items()}, "tries": ad.tries.cpu().numpy(),
+               "flag": ad.flag.cpu().numpy()}  # fmt: skip
+    other = with_table(orc, drc).call(inp["x"], inp["T"], inp["P"], inp["dt"], inp["h"], J=inp["J"], lm=lm)
+    compare("lm and another rate table", got, other, inp["x"], net.atoms)
+
+
+@pytest.mark.gpu
+def test_multiplier_tangent_is_the_time_scaling(gpu):
+    # the call with every coefficient times e^s is the call over e^s dt from e^s h, its h out times e^-s (the
+    # identity above, for any factor; along that curve every error and step factor is unchanged, so no decision
+    # moves): the tangent of x along lm = 1 is its tangent along (ddt, dh) = (dt, h), and that of h is the latter's
+    # minus h, under the adaptive contract, reached through other arithmetic (the rates' tangents, not the step
+    # sizes'). Cells that have not settled (the inputs' own states, dt 300 s), so that x moves along the direction.
+    from maxichem_amd.kincuda_ad import ADSolver
+
+    net, _, inp = cold_case()
+    args = [t(inp[k]) for k in ("x", "T", "P", "dt", "h")]
+    zx, z = zeros_like_inputs(inp)
+    with ADSolver(net, "cuda", photo=True) as ad:
+        a = ad.jvp(*args, zx, z, z, z, z, dlm=torch.ones(CELLS, ad.nr, dtype=torch.float64, device="cuda"),
+                   J=t(inp["J"]))  # fmt: skip
+        b = ad.jvp(*args, zx, z, z, args[3], args[4], J=t(inp["J"]))
+        flag = ad.flag.cpu().numpy()
+    ax, bx, ah, bh, h = (v.cpu().numpy() for v in (a.dx, b.dx, a.dh, b.dh, b.h))
+    moves = np.abs(bx).max(1) >= MOVES
+    ok = (flag == 0) & moves
+    err = np.abs(ax - bx).max(1)[ok] / np.abs(bx).max(1)[ok]
+    errh = np.abs(ah - (bh - h))[ok] / np.abs(bh - h)[ok]
+    print(f"{int(ok.sum())} of {CELLS} cells move and are unflagged; largest difference of the two tangents: x "
+          f"{err.max():.2e}, h {errh.max():.2e}")  # fmt: skip
+    assert ok.sum() >= CELLS // 2 and err.max() <= ADAPT_TOLERANCE and errh.max() <= ADAPT_TOLERANCE
+
+
+@pytest.mark.gpu
+def test_rate_tangents_match_the_differences_of_the_call(gpu):
+    # the tangent along lm and rc's Arrhenius columns (the laws' tangents and the collision limit's share) against
+    # the oracle's whole call with lm + e dlm and the table moved by e drc, on the cells the perturbations leave
+    # smooth and whose differences converge (as the adaptive test above)
+    from maxichem_amd.kincuda_ad import ADSolver
+
+    net, orc, inp = cold_case()
+    lm, dlm, drc = rate_directions(orc, 19)
+
+    def call(e):
+        return with_table(orc, drc, e).call(inp["x"], inp["T"], inp["P"], inp["dt"], inp["h"], J=inp["J"],
+                                             lm=lm + e * dlm)  # fmt: skip
+
+    base, keep, fd = call_differences(call, RATE_EPS)
+    (x1, h1), (x2, h2) = fd
+    args = [t(inp[k]) for k in ("x", "T", "P", "dt", "h")]
+    zx, z = zeros_like_inputs(inp)
+    with ADSolver(net, "cuda", photo=True) as ad:
+        r = ad.jvp(*args, zx, z, z, z, z, lm=t(lm), dlm=t(dlm), drc=t(drc), J=t(inp["J"]))
+        tries = ad.tries.cpu().numpy()
+    dxg, dhg = r.dx.cpu().numpy(), r.dh.cpu().numpy()
+    scale = np.abs(x1).max(1)
+    use = keep & (scale >= MOVES) & (np.abs(x1 - x2).max(1) <= RATE_AGREE * scale) & (tries == base["tries"])
+    use &= r.steps.cpu().numpy() == base["steps"]
+    err = np.abs(dxg - x1).max(1)[use] / scale[use]
+    use_h = use & (np.abs(h1 - h2) <= RATE_AGREE * np.abs(h1))  # away from the step law's kinks, as above
+    errh = np.abs(dhg - h1)[use_h] / np.abs(h1[use_h])
+    print(f"{int(use.sum())} of {CELLS} cells compared for x, {int(use_h.sum())} for h; largest tangent error of x "
+          f"{err.max():.2e}, of h {errh.max():.2e}")  # fmt: skip
+    assert use.sum() >= CELLS // 3 and use_h.sum() >= CELLS // 4
+    assert err.max() <= RATE_TOLERANCE and errh.max() <= RATE_TOLERANCE
+
+
+@pytest.mark.gpu
+def test_vjp_rate_cotangents_are_the_transpose(gpu):
+    # vjp's cotangent of lm per cell, <gx, dx> + gh dh of jvp(dlm) = <g.lm, dlm> (on the cells that move along dlm),
+    # and of rc summed over the cells, sum of <gx, dx> + gh dh of jvp(drc) over the cells = <g.rc, drc> (the cells
+    # the guard lists, whose tangents are NaN, left out of both sides)
+    from maxichem_amd.kincuda_ad import ADSolver
+
+    net, orc, inp = cold_case(seed=9)
+    lm, dlm, drc = rate_directions(orc, 23)
+    args = [t(inp[k]) for k in ("x", "T", "P", "dt", "h")]
+    zx, z = zeros_like_inputs(inp)
+    rng = np.random.default_rng(29)
+    gx, gh = rng.standard_normal((CELLS, len(net.species))), rng.standard_normal(CELLS) / inp["h"]
+    with ADSolver(net, "cuda", photo=True) as ad:
+        v = ad.vjp(*args, t(gx), gh=t(gh), lm=t(lm), J=t(inp["J"]))
+        flag = ad.flag.cpu().numpy()
+        a = ad.jvp(*args, zx, z, z, z, z, lm=t(lm), dlm=t(dlm), J=t(inp["J"]))
+        b = ad.jvp(*args, zx, z, z, z, z, lm=t(lm), drc=t(drc), J=t(inp["J"]))
+    for name in ("x", "h", "done", "steps"):
+        assert torch.equal(getattr(v, name), getattr(a, name)), name
+    g = v.cotangents
+    glm, grc = g.lm.cpu().numpy(), g.rc.cpu().numpy()
+    assert glm.shape == (CELLS, orc.tab.nr) and grc.shape == (orc.tab.nr, ARRHENIUS)
+    assert not grc[orc.tab.nrt:].any()  # a photolysis branch has no law
+
+    def out(r):
+        dx, dh = r.dx.cpu().numpy(), r.dh.cpu().numpy()
+        return (gx * dx).sum(1) + gh * dh, np.abs(gx * dx).sum(1) + np.abs(gh * dh), np.abs(dx).max(1) >= MOVES
+
+    left, lscale, moves = out(a)
+    ok = (flag == 0) & moves
+    right = (glm * dlm).sum(1)
+    err = np.abs(left - right)[ok] / (lscale + np.abs(glm * dlm).sum(1))[ok]
+    left_rc, lscale_rc, _ = out(b)
+    ok_rc = flag == 0
+    total, rtotal = left_rc[ok_rc].sum(), (grc * drc).sum()
+    err_rc = abs(total - rtotal) / (lscale_rc[ok_rc].sum() + np.abs(grc * drc).sum())
+    print(f"{int(ok.sum())} of {CELLS} cells move along dlm and are unflagged; largest dot-product mismatch of lm "
+          f"{err.max():.2e}; of rc (summed over the {int(ok_rc.sum())} unflagged cells) {err_rc:.2e}")  # fmt: skip
+    assert ok.sum() >= CELLS // 3 and err.max() <= DOT_TOLERANCE_ADAPTIVE and err_rc <= DOT_TOLERANCE_ADAPTIVE

@@ -129,10 +129,20 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixe
     return "\n".join(lines) + "\n"
 
 
+def _jvp_region(tab: Tables) -> int:
+    jsize = 2 * tab.nr + 4 * (tab.ns + 1) + 4 * (tab.nr + 1) + 4 * tab.npart
+    return jsize + (jsize & 1)
+
+
 def jvp_lds_bytes(tab: Tables, cpw: int) -> int:
     """Dynamic LDS of a tangent workgroup (kernels/jvp.hip): each cell's call region and its tangent region."""
-    jsize = 2 * tab.nr + 4 * (tab.ns + 1) + 4 * (tab.nr + 1) + 4 * tab.npart
-    return 8 * (tab.layout()["SIZE"] + jsize + (jsize & 1)) * cpw
+    return 8 * (tab.layout()["SIZE"] + _jvp_region(tab)) * cpw
+
+
+def jvp_recover_lds_bytes(tab: Tables) -> int:
+    """Dynamic LDS of a tangent recovery workgroup (mxc_jvp_recover): the recovery's, with the tangent region after
+    the cell's call region."""
+    return lds_bytes(tab, 1)[1] + 8 * _jvp_region(tab)
 
 
 def lds_bytes(tab: Tables, cpw: int, mixed: bool = False, rates_global: bool = False) -> tuple[int, int]:

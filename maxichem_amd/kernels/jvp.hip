@@ -455,8 +455,8 @@ __device__ __forceinline__ double step_factor_tangent(double en, double end, boo
 // accepted try (adapt 0: the frozen contract), or through every try with the step control's tangents (adapt 1: the
 // adaptive contract, with the tangents of dt and h in dtd, hd, and of the result's h into hdo), then the guard with
 // its tangent. Inputs' tangents: x0d [B][NS], Td, Pd [B], Jd [B][NJ] (may be null).
-// Output: xd [B][NS], the tangent of the guard's result (network order); a cell the guard fails gets NaN there (the
-// recovery recomputes it without a tangent).
+// Output: xd [B][NS], the tangent of the guard's result (network order); a cell the guard fails gets NaN there, and
+// mxc_jvp_recover then recomputes it with its tangent.
 extern "C" __global__ void __launch_bounds__(WAVE* CPW)
     mxc_jvp(NetP __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
             const double* __restrict__ Pa, const double* __restrict__ dta, double* __restrict__ hio,
@@ -647,17 +647,227 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
   }
 }
 
-// The recovery of the cells the tangent kernel lists, with the cells' log rate multipliers lm [B][NR] (the call's
-// recovery otherwise, bit for bit).
+// y <- W^-1 y for a lead / trailing vector pair, with the recovery's dense factors (through the LDS vector yv).
+__device__ __forceinline__ void dense_solve_pair(const double* W, const int* pr, const double* dinv, double* yv,
+                                                 double& vl, double& vt, int lane) {
+  to_lds(yv, vl, vt, lane);
+  dense_solve(W, pr, dinv, yv, lane);
+  from_lds(yv, vl, vt, lane);
+}
+
+// The recovery of the cells the tangent kernel listed (mxc_recover's passes, bit for bit: the same device functions
+// in the same order, with the multipliers lm), with the tangent carried through the pass whose result is the cell's:
+// each pass starts again from the inputs' tangents; each try's stage tangents solve the dense W of the try
+// (W Kdot = bdot + Jdot K, plus the step's terms under the adaptive contract) on its threshold-pivoted factors. A
+// pass's first step min(h, dt split^k) takes the tangent of its smaller side (half of each at a tie), then that of
+// max(., H_MIN). The result's tangent goes through the guard's, as the tangent kernel's; a cell that still fails the
+// guard (flag 2) gets NaN. Workgroups of REC_WAVES wavefronts share each dense LU as mxc_recover's.
 extern "C" __global__ void __launch_bounds__(WAVE* REC_WAVES)
-    mxc_recover_lm(NetP __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
-                   const double* __restrict__ Pa, const double* __restrict__ dta, const double* __restrict__ hin,
-                   double* __restrict__ xout, double* __restrict__ hio, int* __restrict__ doneo,
-                   double* __restrict__ noko, int* __restrict__ redo, const int* __restrict__ list,
-                   int* __restrict__ count, double* __restrict__ tries, const double* __restrict__ Jin, double rtol,
-                   double atol, int max_steps, double* __restrict__ rscratch, const double* __restrict__ lm) {
-  recover_cells(net, x0, Ta, Pa, dta, hin, xout, hio, doneo, noko, redo, list, count, tries, Jin, rtol, atol,
-                max_steps, rscratch, lm);
+    mxc_jvp_recover(NetP __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
+                    const double* __restrict__ Pa, const double* __restrict__ dta, const double* __restrict__ hin,
+                    double* __restrict__ xout, double* __restrict__ hio, int* __restrict__ doneo,
+                    double* __restrict__ noko, int* __restrict__ redo, const int* __restrict__ list,
+                    int* __restrict__ count, double* __restrict__ tries, const double* __restrict__ Jin, double rtol,
+                    double atol, int max_steps, const double* __restrict__ lm, const double* __restrict__ x0d,
+                    const double* __restrict__ Td, const double* __restrict__ Pd, const double* __restrict__ Jd,
+                    double* __restrict__ xd, int adapt, const double* __restrict__ dtd, const double* __restrict__ hd,
+                    double* __restrict__ hdo, const double* __restrict__ lmd, const double* __restrict__ rcd) {
+  const int nfl = count[COUNT_RECOMPUTED];
+  if ((int)blockIdx.x >= nfl) return;
+  extern __shared__ double lds[];
+  const int lane = (int)(threadIdx.x & (WAVE - 1)), wv = (int)(threadIdx.x / WAVE);
+  double* ws = lds;
+  double* ts = ws + L_SIZE;
+  double* rk = ws + L_KF;
+  double* rkd = ts + J_KD;
+  double* W = ts + J_SIZE;
+  double* dinv = W + NS * NS;
+  double* yv = dinv + NS;
+  int* pr = (int*)(yv + NS);
+  volatile int* cmd = pr + NS;
+  if (wv != 0) {
+    for (;;) {
+      __syncthreads();
+      if (*cmd == 0) return;
+      dense_lu(W, pr, dinv, lane, wv);
+    }
+  }
+  double c[NT];
+  LeadRow a21;
+  const bool ad = adapt != 0;
+  int q = (int)blockIdx.x;
+  while (q < nfl) {
+    const int cell = list[q];
+    const double Tc = Ta[cell], dt = dta[cell], nd = Pa[cell] / (KBOLTZ * Tc);
+    const double* xr = x0 + (size_t)cell * NS;
+    const double* xdr = x0d + (size_t)cell * NS;
+    const double* Jc = NJ ? Jin + (size_t)cell * NJ : nullptr;
+    const double* lmc = lm ? lm + (size_t)cell * NR : nullptr;
+    cell_rates(ws, rk, Tc, nd, net, Jc, lane, lmc);
+    const double Tdc = Td ? Td[cell] : 0.0, Pdc = Pd ? Pd[cell] : 0.0;
+    const double ndd = nd * (Pdc / Pa[cell] - Tdc / Tc);
+    rates_tangent(ts, rkd, Tc, Tdc, nd, ndd, net, Jc, (NJ && Jd) ? Jd + (size_t)cell * NJ : nullptr, lmc,
+                  lmd ? lmd + (size_t)cell * NR : nullptr, rcd, lane);
+    const double h0 = isfinite(hin[cell]) ? hin[cell] : dt;
+    const double m = fmin(h0, dt);
+    const double dtdc = (ad && dtd) ? dtd[cell] : 0.0;
+    const double h0d = !ad ? 0.0 : (isfinite(hin[cell]) ? (hd ? hd[cell] : 0.0) : dtdc);
+    const double md = h0 < dt ? h0d : (h0 > dt ? dtdc : 0.5 * (h0d + dtdc));  // m's tangent
+    int ntry = 0;
+    for (int pass = 0;; pass++) {
+      const gptr<int> perm = opaque(net)->perm;
+      double Nl = lane < K0 ? xr[perm[lane]] : 0.0, Nt = lane < NT ? xr[perm[K0 + lane]] : 0.0;
+      double Ndl = lane < K0 ? xdr[perm[lane]] : 0.0, Ndt = lane < NT ? xdr[perm[K0 + lane]] : 0.0;
+      const double split = pow(REC_RETRY_SPLIT, (double)pass), cap = dt * split;
+      const double start = pass == 0 ? m : fmin(m, cap);
+      const double sd = pass == 0 || cap > m ? md : (cap < m ? dtdc * split : 0.5 * (md + dtdc * split));
+      double hdot = !ad ? 0.0 : (start > H_MIN ? sd : (start < H_MIN ? 0.0 : 0.5 * sd));
+      double tdot = 0.0, pendot = 0.0, phadot = 0.0;
+      Step s = {fmax(start, H_MIN), 0.0, 0.0, 0.0, 0.0, false};
+      int it = dt > 0.0 ? 0 : max_steps;
+      while (it < max_steps && s.t < dt * CALL_END) {
+        const double ha = fmin(s.h, dt - s.t), igh = 1.0 / (GAMMA * ha), ih = 1.0 / ha;
+        const double rem = dt - s.t;
+        const double had = ad ? (s.h < rem ? hdot : (s.h > rem ? dtdc - tdot : 0.5 * (hdot + dtdc - tdot))) : 0.0;
+        double Fl = 0.0, Ft = 0.0, ul = 0.0, ut = 0.0, u0l = 0.0, u0t = 0.0;
+        const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
+        jacobian(c, a21, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
+        // W = I / (gamma h) - Jn - w 1^T, dense (recover.cuh:124-130)
+        to_lds(yv, ul, ut, lane);
+        for (int e = lane; e < K0 * NS; e += WAVE) {
+          const int i = e / NS;
+          W[e] = (e == i * (NS + 1) ? ws[e] + igh : ws[e]) - yv[i];
+        }
+        if (lane < NT) {
+          double* row = W + (K0 + lane) * NS;
+#pragma unroll
+          for (int k = 0; k < K0; k++) row[k] = a21[k] - ut;
+#pragma unroll
+          for (int j = 0; j < NT; j++) row[K0 + j] = (j == lane ? c[j] + igh : c[j]) - ut;
+        }
+        if (lane == 0) *cmd = 1;
+        __syncthreads();
+        dense_lu(W, pr, dinv, lane, 0);
+        double K1l, K1t, K2l, K2t, K3l, K3t, K4l, K4t;
+        to_lds(yv, Fl, Ft, lane);
+        dense_solve(W, pr, dinv, yv, lane);
+        from_lds(yv, K1l, K1t, lane);
+        to_lds(yv, fma(RODAS_C21 * ih, K1l, Fl), fma(RODAS_C21 * ih, K1t, Ft), lane);
+        dense_solve(W, pr, dinv, yv, lane);
+        from_lds(yv, K2l, K2t, lane);
+        double Yl = Nl + 2.0 * K1l, Yt = Nt + 2.0 * K1t;
+        rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
+        to_lds(yv, fma(ih, K1l - K2l, Fl), fma(ih, K1t - K2t, Ft), lane);
+        dense_solve(W, pr, dinv, yv, lane);
+        from_lds(yv, K3l, K3t, lane);
+        Yl = Nl + 2.0 * K1l + K3l;
+        Yt = Nt + 2.0 * K1t + K3t;
+        rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
+        to_lds(yv, fma(ih, fma(-RODAS_C43, K3l, K1l - K2l), Fl), fma(ih, fma(-RODAS_C43, K3t, K1t - K2t), Ft), lane);
+        dense_solve(W, pr, dinv, yv, lane);
+        from_lds(yv, K4l, K4t, lane);
+        bool ok = false;
+        const bool first = s.t == 0.0;
+        const double en = try_error(Nl, Nt, K1l, K1t, K3l, K3t, K4l, K4t, Yl, Yt, first, rtol, atol, lane, ok);
+        double end = 0.0;  // the error's tangent (adaptive contract)
+        if (ok || ad) {
+          // the try's tangent on its dense factors, as the tangent kernel's on the sparse ones
+          double Fel, Fet, Gl, Gt, Ql, Qt, dl, dtt;
+          double K1dl, K1dt, K2dl, K2dt, K3dl, K3dt, K4dl, K4dt;
+          const double sg = had * ih * ih / GAMMA, hq = had * ih * ih;
+          rhs_tangent<true>(Nl, Nt, Ndl, Ndt, K1l, K1t, Fel, Fet, Gl, Gt, net, nd, ndd, ts, rk, rkd, lane);
+          K1dl = Fel + Gl;
+          K1dt = Fet + Gt;
+          if (ad) {
+            K1dl = fma(sg, K1l, K1dl);
+            K1dt = fma(sg, K1t, K1dt);
+          }
+          dense_solve_pair(W, pr, dinv, yv, K1dl, K1dt, lane);
+          rhs_tangent<true>(Nl, Nt, Ndl, Ndt, K2l, K2t, Ql, Qt, Gl, Gt, net, nd, ndd, ts, rk, rkd, lane);
+          K2dl = fma(RODAS_C21 * ih, K1dl, Fel) + Gl;
+          K2dt = fma(RODAS_C21 * ih, K1dt, Fet) + Gt;
+          if (ad) {
+            K2dl += sg * K2l - hq * RODAS_C21 * K1l;
+            K2dt += sg * K2t - hq * RODAS_C21 * K1t;
+          }
+          dense_solve_pair(W, pr, dinv, yv, K2dl, K2dt, lane);
+          const double Y3el = Ndl + 2.0 * K1dl, Y3et = Ndt + 2.0 * K1dt;
+          rhs_tangent<false>(Nl + 2.0 * K1l, Nt + 2.0 * K1t, Y3el, Y3et, 0.0, 0.0, Ql, Qt, dl, dtt, net, nd, ndd, ts,
+                             rk, rkd, lane);
+          rhs_tangent<true>(Nl, Nt, Ndl, Ndt, K3l, K3t, dl, dtt, Gl, Gt, net, nd, ndd, ts, rk, rkd, lane);
+          K3dl = fma(ih, K1dl - K2dl, Ql) + Gl;
+          K3dt = fma(ih, K1dt - K2dt, Qt) + Gt;
+          if (ad) {
+            K3dl += sg * K3l - hq * (K1l - K2l);
+            K3dt += sg * K3t - hq * (K1t - K2t);
+          }
+          dense_solve_pair(W, pr, dinv, yv, K3dl, K3dt, lane);
+          rhs_tangent<false>(Nl + 2.0 * K1l + K3l, Nt + 2.0 * K1t + K3t, Y3el + K3dl, Y3et + K3dt, 0.0, 0.0, Ql, Qt,
+                             dl, dtt, net, nd, ndd, ts, rk, rkd, lane);
+          rhs_tangent<true>(Nl, Nt, Ndl, Ndt, K4l, K4t, dl, dtt, Gl, Gt, net, nd, ndd, ts, rk, rkd, lane);
+          K4dl = fma(ih, fma(-RODAS_C43, K3dl, K1dl - K2dl), Ql) + Gl;
+          K4dt = fma(ih, fma(-RODAS_C43, K3dt, K1dt - K2dt), Qt) + Gt;
+          if (ad) {
+            K4dl += sg * K4l - hq * fma(-RODAS_C43, K3l, K1l - K2l);
+            K4dt += sg * K4t - hq * fma(-RODAS_C43, K3t, K1t - K2t);
+          }
+          dense_solve_pair(W, pr, dinv, yv, K4dl, K4dt, lane);
+          const double ydl = Ndl + 2.0 * K1dl + K3dl + K4dl, ydt = Ndt + 2.0 * K1dt + K3dt + K4dt;
+          if (ad)
+            end = error_tangent(Nl, Nt, Yl, Yt, K4l, K4t, Ndl, Ndt, ydl, ydt, K4dl, K4dt, first, en, rtol, atol, lane);
+          if (ok) {
+            Ndl = lane < K0 && Yl >= 0.0 ? ydl : 0.0;
+            Ndt = lane < NT && Yt >= 0.0 ? ydt : 0.0;
+          }
+        }
+        if (ok) {
+          Nl = fmax(Yl, 0.0);
+          Nt = fmax(Yt, 0.0);
+          s.t += ha;
+          s.nok += 1.0;
+          tdot += had;
+        }
+        const double fac = step_factor(en, ok, s.prej, s.pen, s.pha, ha);
+        if (ad) {
+          hdot = fma(had, fac, ha * step_factor_tangent(en, end, ok, s.prej, s.pen, pendot, s.pha, phadot, ha, had));
+          pendot = end;
+          phadot = had;
+        }
+        s.h = ha * fac;
+        s.prej = !ok;
+        s.pen = en;
+        s.pha = ha;
+        it++;
+        ntry++;
+      }
+      // the guard's test of the result, which goes to the cell's output row (a later pass overwrites it)
+      const int dn = s.t >= dt * CALL_END ? 1 : 0;
+      const int code = __builtin_amdgcn_readfirstlane(
+          guard_wave(Nl, Nt, xr, xout + (size_t)cell * NS, dn != 0, s.nok, atol, net, lane));
+      if (code == GUARD_DRIFT && pass < REC_RETRIES) continue;  // only drifted: again, with a smaller first try
+      double* xdo = xd + (size_t)cell * NS;
+      if (code == 0) {
+        guard_tangent(Nl, Nt, Ndl, Ndt, xr, xdr, xdo, net, lane);
+      } else {
+        for (int i = lane; i < NS; i += WAVE) xdo[i] = DNAN;
+      }
+      int nx = 0;
+      if (lane == 0) {
+        hio[cell] = s.h;
+        if (hdo) hdo[cell] = !ad ? 0.0 : (code == 0 ? hdot : DNAN);
+        doneo[cell] = code ? 0 : dn;  // done: finished and passing the guard
+        noko[cell] = s.nok;
+        redo[cell] = code ? 2 : 1;
+        if (tries) tries[cell] += (double)ntry;
+        atomicAdd(count + (code ? COUNT_STILL_FAILED : COUNT_RECOVERED), 1);
+        nx = (int)gridDim.x + atomicAdd(count + COUNT_RECOVERY_QUEUE, 1);
+      }
+      q = __builtin_amdgcn_readfirstlane(nx);
+      break;
+    }
+  }
+  if (lane == 0) *cmd = 0;
+  __syncthreads();  // the other wavefronts leave
 }
 
 // The rate table's cotangent grc [NR][9] (its nine Arrhenius columns) from the cells' cotangents of their log rate

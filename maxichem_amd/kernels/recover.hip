@@ -154,16 +154,14 @@ __device__ __forceinline__ void from_lds(const double* y, double& vl, double& vt
 
 // A workgroup of REC_WAVES wavefronts per listed cell: wavefront 0 steps the cell; the others wait at a workgroup
 // barrier for each of its dense factorizations (cmd 1: factor W, 0: the workgroup's cells are done), so every
-// wavefront passes the same barriers. lm: the cells' log rate multipliers [B][NR] or null (cell_rates).
-__device__ __forceinline__ void recover_cells(NetP __restrict__ net, const double* __restrict__ x0,
-                                              const double* __restrict__ Ta, const double* __restrict__ Pa,
-                                              const double* __restrict__ dta, const double* __restrict__ hin,
-                                              double* __restrict__ xout, double* __restrict__ hio,
-                                              int* __restrict__ doneo, double* __restrict__ noko,
-                                              int* __restrict__ redo, const int* __restrict__ list,
-                                              int* __restrict__ count, double* __restrict__ tries,
-                                              const double* __restrict__ Jin, double rtol, double atol, int max_steps,
-                                              double* __restrict__ rscratch, const double* __restrict__ lm) {
+// wavefront passes the same barriers.
+extern "C" __global__ void __launch_bounds__(WAVE* REC_WAVES)
+    mxc_recover(NetP __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
+                const double* __restrict__ Pa, const double* __restrict__ dta, const double* __restrict__ hin,
+                double* __restrict__ xout, double* __restrict__ hio, int* __restrict__ doneo,
+                double* __restrict__ noko, int* __restrict__ redo, const int* __restrict__ list,
+                int* __restrict__ count, double* __restrict__ tries, const double* __restrict__ Jin, double rtol,
+                double atol, int max_steps, double* __restrict__ rscratch) {
   const int nfl = count[COUNT_RECOMPUTED];
   if ((int)blockIdx.x >= nfl) return;
   extern __shared__ double lds[];
@@ -195,7 +193,7 @@ __device__ __forceinline__ void recover_cells(NetP __restrict__ net, const doubl
     const double* xr = x0 + (size_t)cell * NS;
     const double* Jc = NJ ? Jin + (size_t)cell * NJ : nullptr;
     // a listed cell passed the call kernel's screen, so it is not screened again
-    cell_rates(ws, rk, Tc, nd, net, Jc, lane, lm ? lm + (size_t)cell * NR : nullptr);
+    cell_rates(ws, rk, Tc, nd, net, Jc, lane);
     const double h0 = isfinite(hin[cell]) ? hin[cell] : dt;
     const double m = fmin(h0, dt);
     int ntry = 0;
@@ -281,15 +279,4 @@ __device__ __forceinline__ void recover_cells(NetP __restrict__ net, const doubl
   }
   if (lane == 0) *cmd = 0;
   __syncthreads();  // the other wavefronts leave
-}
-
-extern "C" __global__ void __launch_bounds__(WAVE* REC_WAVES)
-    mxc_recover(NetP __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
-                const double* __restrict__ Pa, const double* __restrict__ dta, const double* __restrict__ hin,
-                double* __restrict__ xout, double* __restrict__ hio, int* __restrict__ doneo,
-                double* __restrict__ noko, int* __restrict__ redo, const int* __restrict__ list,
-                int* __restrict__ count, double* __restrict__ tries, const double* __restrict__ Jin, double rtol,
-                double atol, int max_steps, double* __restrict__ rscratch) {
-  recover_cells(net, x0, Ta, Pa, dta, hin, xout, hio, doneo, noko, redo, list, count, tries, Jin, rtol, atol,
-                max_steps, rscratch, nullptr);
 }

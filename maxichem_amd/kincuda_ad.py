@@ -7,7 +7,8 @@ tangent, rejected ones included, the error's and the step factor's, so the tange
 result's h has one. adapt=False (the frozen contract) holds the accepted step sizes fixed: rejected tries drop out, no
 tangent through dt or h, and none of h out. The tangents of x, T, P and J enter under both. The call's result is
 CudaSolver.launch's, bit for bit (the same kernels' arithmetic in the same order). A cell the guard fails is
-recomputed by the recovery without a tangent: its tangents are NaN, its flag 1 or 2 as in launch.
+recomputed by the recovery with its tangent (mxc_jvp_recover, on the recovery's dense factors), its flag 1 or 2 as in
+launch; a cell that still fails (flag 2) has NaN tangents.
 
 Beside them, both contracts take per-cell log rate multipliers lm [B, R] (a reaction's two coefficients times exp(lm)
 after the collision limit, a photolysis branch's kf = J exp(lm)) and another rate table rc [R, W] for the call, with
@@ -96,8 +97,14 @@ class ADSolver(CudaSolver):
                 raise ValueError(f"one cell's tangent kernel needs {self.lds_jvp} B of LDS; the device has "
                                  f"{info['lds_per_block']}")  # fmt: skip
             self.max_blocks_jvp = info["compute_units"] * max(1, _hip.occupancy(self._jvp, WAVE, self.lds_jvp))
-            self._recover_lm = self._module.function("mxc_recover_lm")
             self._rc_cotangent = self._module.function("mxc_rc_cotangent")
+            self._jvp_recover = self._module.function("mxc_jvp_recover")
+            self.lds_jvp_rec = _kernels.jvp_recover_lds_bytes(self.tab)
+            if self.lds_jvp_rec > info["lds_per_block"]:
+                raise ValueError(f"one cell's tangent recovery needs {self.lds_jvp_rec} B of LDS; the device has "
+                                 f"{info['lds_per_block']}")  # fmt: skip
+            self.rec_blocks_jvp = info["compute_units"] * max(
+                1, _hip.occupancy(self._jvp_recover, WAVE * _kernels.REC_WAVES, self.lds_jvp_rec))
         # the rate table's width as the reference gives it: the nine Arrhenius columns, and the Troe parameters and
         # the efficiency when a reaction has them
         self.rcw = self.tab.rc.shape[1] if self.tab.has_troe or self.tab.has_eff else ARRHENIUS_COLUMNS
@@ -136,7 +143,8 @@ class ADSolver(CudaSolver):
         differentiates the step control too: every try's tangent, the error's and the step factor's, so ddt and dh
         enter and the result's h has a tangent; adapt=False holds the accepted steps fixed (ddt and dh do not enter,
         the result's dh is zero). The result's x, h, done and steps are CudaSolver.launch's; dx, dh the tangents of x
-        and h (NaN for a cell the guard lists). Refusals and counts as launch's.
+        and h (those of a recomputed cell through the recovery's tries; NaN for one that still fails, flag 2).
+        Refusals and counts as launch's.
 
         lm [B, R]: per-cell log rate multipliers (a reaction's two coefficients times exp(lm) after the collision
         limit, a photolysis branch's kf = J exp(lm)); a non-finite one makes its cell bad input. rc [R, W]: the rate
@@ -205,13 +213,13 @@ class ADSolver(CudaSolver):
                 p(dh), p(hd), p(lm), p(dlm), p(drc)]  # fmt: skip
         if B:
             _hip.launch(self._jvp, max(1, min(B, self.max_blocks_jvp)), WAVE, self.lds_jvp, stream, args)
-        # the cells the guard failed: the recovery's result, without a tangent (NaN from the tangent kernel)
+        # the cells the guard failed: the recovery's result with its tangent
         rec_args = [p(net), p(x), p(T), p(P), p(dt), p(h), p(xo), p(ho), p(done), p(nok), p(redo), p(lst),
                     p(self.count), p(self.tries), p(jx), ctypes.c_double(rtol), ctypes.c_double(atol),
-                    ctypes.c_int(REC_STEPS * max_steps), p(self._rscratch)]  # fmt: skip
-        recover = self._recover if lm is None else self._recover_lm
-        _hip.launch(recover, max(1, min(B, self.rec_blocks)), WAVE * _kernels.REC_WAVES, self.lds_rec, stream,
-                    rec_args + ([] if lm is None else [p(lm)]))  # fmt: skip
+                    ctypes.c_int(REC_STEPS * max_steps), p(lm), p(dx), p(dT), p(dP), p(dJ), p(xd),
+                    ctypes.c_int(int(bool(adapt))), p(ddt), p(dh), p(hd), p(dlm), p(drc)]  # fmt: skip
+        _hip.launch(self._jvp_recover, max(1, min(B, self.rec_blocks_jvp)), WAVE * _kernels.REC_WAVES,
+                    self.lds_jvp_rec, stream, rec_args)  # fmt: skip
         self._status = status
         return xo, ho, done, nok, redo
 
@@ -228,8 +236,8 @@ class ADSolver(CudaSolver):
         reduced from the multipliers' by mxc_rc_cotangent (a multiplier and the law enter a reaction's coefficients
         alike, the law's share 1 - wA of the collision limit). It costs S + 2 + branches (+ 2) tangent calls, and the
         thermal reactions' count more with rates (rates=False leaves the cotangents of lm and rc None). Under the
-        frozen contract the cotangents of dt and h are zero. A cell the guard lists has NaN cotangents, its flag as
-        launch's, and is left out of the rate table's sum (fallback["recomputed"] counts them)."""
+        frozen contract the cotangents of dt and h are zero. A cell that still fails the guard (flag 2) has NaN
+        cotangents and is left out of the rate table's sum (fallback["still_failed"] counts them)."""
         self.flag = None
         self._require_open()
         B = check_inputs(self.dev, self.ns, x, T, P, dt, h, rtol, atol, max_steps)
@@ -287,7 +295,7 @@ class ADSolver(CudaSolver):
                     g.lm[:, nrt:] = jx * g.J
             xo, ho, done, nok, redo = result
             if rates:
-                skip = redo.to(torch.int32)  # the recomputed cells have no tangent: left out of the table's sum
+                skip = (redo == 2).to(torch.int32)  # a cell that still fails has no tangent: left out of the sum
                 stream = torch.cuda.current_stream(self.dev).cuda_stream
                 _hip.launch(self._rc_cotangent, self.nr, WAVE, 0, stream,
                             [_hip.ptr(net), _hip.ptr(T), _hip.ptr(P), _hip.ptr(g.lm), _hip.ptr(skip), ctypes.c_int(B),

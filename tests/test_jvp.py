@@ -16,6 +16,8 @@ against the call, tests/test_kernel_copies.py).
   the tangent kernel's primal, the tangent along lm = 1 is the tangent along (dt, h) = (dt, h), the tangents along
   lm and the Arrhenius columns of rc match the whole call's differences (the oracle with lm and another table), and
   vjp's cotangents of lm (per cell) and rc (summed over the cells) pass the dot-product test.
+- The recovery: cells the call leaves unfinished are recomputed with their tangent; results and flags are launch's
+  bit for bit, and the tangent matches the oracle's whole-call differences through its recovery.
 """
 
 import dataclasses
@@ -216,15 +218,16 @@ def adaptive_differences(orc, inp, d):
     return call_differences(call)
 
 
-def call_differences(call, steps=ADAPT_EPS):
-    """adaptive_differences of call(e), the oracle's call at e along a direction, at the two steps."""
+def call_differences(call, steps=ADAPT_EPS, recovered: bool = False):
+    """adaptive_differences of call(e), the oracle's call at e along a direction, at the two steps (recovered: the
+    cells the recovery passed count too)."""
     base = call(0.0)
-    keep = base["flag"] == 0
+    keep = (base["flag"] == 0) | (recovered & (base["flag"] == 1))
     fd = []
     for e in steps:
         up, down = call(e), call(-e)
         for r in (up, down):
-            keep &= (r["tries"] == base["tries"]) & (r["steps"] == base["steps"]) & (r["flag"] == 0)
+            keep &= (r["tries"] == base["tries"]) & (r["steps"] == base["steps"]) & (r["flag"] == base["flag"])
         fd.append(((up["x"] - down["x"]) / (2 * e), (up["h"] - down["h"]) / (2 * e)))
     return base, keep, fd
 
@@ -472,3 +475,64 @@ def test_vjp_rate_cotangents_are_the_transpose(gpu):
     print(f"{int(ok.sum())} of {CELLS} cells move along dlm and are unflagged; largest dot-product mismatch of lm "
           f"{err.max():.2e}; of rc (summed over the {int(ok_rc.sum())} unflagged cells) {err_rc:.2e}")  # fmt: skip
     assert ok.sum() >= CELLS // 3 and err.max() <= DOT_TOLERANCE_ADAPTIVE and err_rc <= DOT_TOLERANCE_ADAPTIVE
+
+
+REC_MAX_STEPS = 30  # the cells that have not settled take up to 300 tries: those above 30 go to the recovery
+
+
+@pytest.mark.gpu
+def test_tangent_through_the_recovery(gpu):
+    # cells the call leaves unfinished (max_steps 30) are recomputed by the recovery with ten times the tries on dense
+    # threshold-pivoted factors: their results are launch's bit for bit, flag 1, and their tangent, carried through
+    # the recovery's tries, matches the oracle's whole-call differences (its recovery too) along directions of every
+    # input, as the other cells' do
+    from maxichem_amd.kincuda_ad import ADSolver
+
+    net, orc, inp = cold_case()
+    lm, dlm, drc = rate_directions(orc, 31)
+    rng = np.random.default_rng(37)
+    x = inp["x"]
+    dx = x * rng.uniform(-1, 1, x.shape)
+    dx -= x * (dx.sum(1, keepdims=True) / x.sum(1, keepdims=True))
+    d = {"dx": dx, "dT": 0.005 * inp["T"] * rng.uniform(-1, 1, CELLS),
+         "dP": 0.005 * inp["P"] * rng.uniform(-1, 1, CELLS), "dJ": 0.5 * inp["J"] * rng.uniform(-1, 1, inp["J"].shape),
+         "ddt": 0.01 * inp["dt"] * rng.uniform(-1, 1, CELLS), "dh": 0.1 * inp["h"] * rng.uniform(-1, 1, CELLS)}
+
+    def call(e):
+        return with_table(orc, drc, e).call(x + e * d["dx"], inp["T"] + e * d["dT"], inp["P"] + e * d["dP"],
+                                             inp["dt"] + e * d["ddt"], inp["h"] + e * d["dh"], J=inp["J"] + e * d["dJ"],
+                                             lm=lm + e * dlm, max_steps=REC_MAX_STEPS)  # fmt: skip
+
+    base, keep, fd = call_differences(call, RATE_EPS, recovered=True)
+    (x1, h1), (x2, h2) = fd
+    args = [t(inp[k]) for k in ("x", "T", "P", "dt", "h")]
+    tangents = [t(d[k]) for k in ("dx", "dT", "dP", "ddt", "dh")]
+    zx, z = zeros_like_inputs(inp)
+    with plain_solver(net) as plain:
+        want = plain.launch(*args, J=t(inp["J"]), max_steps=REC_MAX_STEPS)
+        want_flag = plain.flag.cpu().numpy()
+    with ADSolver(net, "cuda", photo=True) as ad:
+        same = ad.jvp(*args, *tangents, J=t(inp["J"]), dJ=t(d["dJ"]), max_steps=REC_MAX_STEPS)
+        assert np.array_equal(ad.flag.cpu().numpy(), want_flag) and (want_flag == 1).sum() >= CELLS // 4
+        for name in ("x", "h", "done", "steps"):
+            assert torch.equal(getattr(same, name), getattr(want, name)), name
+        r = ad.jvp(*args, *tangents, lm=t(lm), dlm=t(dlm), drc=t(drc), J=t(inp["J"]), dJ=t(d["dJ"]),
+                   max_steps=REC_MAX_STEPS)  # fmt: skip
+        flag = ad.flag.cpu().numpy()
+        tries = ad.tries.cpu().numpy()
+    dxg, dhg = r.dx.cpu().numpy(), r.dh.cpu().numpy()
+    assert np.array_equal(flag, base["flag"])
+    assert np.isfinite(dxg[flag != 2]).all() and np.isfinite(dhg[flag != 2]).all()
+    scale = np.abs(x1).max(1)
+    use = keep & (np.abs(x1 - x2).max(1) <= RATE_AGREE * scale) & (tries == base["tries"])
+    use &= r.steps.cpu().numpy() == base["steps"]
+    err = np.abs(dxg - x1).max(1) / np.maximum(scale, 1e-300)
+    use_h = use & (np.abs(h1 - h2) <= RATE_AGREE * np.abs(h1))
+    errh = np.abs(dhg - h1) / np.maximum(np.abs(h1), 1e-300)
+    rec = flag == 1
+    print(f"{int((use & rec).sum())} recovered and {int((use & ~rec).sum())} other cells compared for x, "
+          f"{int((use_h & rec).sum())} recovered for h; largest tangent error of x on the recovered "
+          f"{err[use & rec].max():.2e}, on the others {err[use & ~rec].max():.2e}; of h on the recovered "
+          f"{errh[use_h & rec].max():.2e}")  # fmt: skip
+    assert (use & rec).sum() >= CELLS // 4 and (use_h & rec).sum() >= CELLS // 8
+    assert err[use].max() <= RATE_TOLERANCE and errh[use_h].max() <= RATE_TOLERANCE

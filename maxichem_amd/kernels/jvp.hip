@@ -454,19 +454,23 @@ __device__ __forceinline__ double step_factor_tangent(double en, double end, boo
 // The call and its tangent: the call kernel's loop (no proposals, FP64) with the tangent carried through each
 // accepted try (adapt 0: the frozen contract), or through every try with the step control's tangents (adapt 1: the
 // adaptive contract, with the tangents of dt and h in dtd, hd, and of the result's h into hdo), then the guard with
-// its tangent. Inputs' tangents: x0d [B][NS], Td, Pd [B], Jd [B][NJ] (may be null).
-// Output: xd [B][NS], the tangent of the guard's result (network order); a cell the guard fails gets NaN there, and
-// mxc_jvp_recover then recomputes it with its tangent.
+// its tangent. D directions in one launch: the persistent queue hands out (direction, cell) pairs, each wavefront
+// repeating its cell's call (the same bits for every direction) with that direction's tangent, so a small batch
+// still fills the GPU. Direction d's tangents: x0d [D][B][NS], Td, Pd, dtd, hd [D][B], Jd [D][B][NJ], lmd [D][B][NR],
+// rcd [D][NR][9] (each may be null: zero). Outputs: xd [D][B][NS], hdo [D][B], the tangents of the guard's result
+// (network order) and of h; the call's results, counts and list come from direction 0's pairs (the other pairs write
+// the same bits to xout). A cell the guard fails gets NaN, and mxc_jvp_recover then recomputes it with its tangent.
 extern "C" __global__ void __launch_bounds__(WAVE* CPW)
     mxc_jvp(NetP __restrict__ net, const double* __restrict__ x0, const double* __restrict__ Ta,
-            const double* __restrict__ Pa, const double* __restrict__ dta, double* __restrict__ hio,
-            double* __restrict__ xout, int* __restrict__ doneo, double* __restrict__ noko, int* __restrict__ status,
+            const double* __restrict__ Pa, const double* __restrict__ dta, const double* __restrict__ hin_a,
+            double* __restrict__ hout, double* __restrict__ xout, int* __restrict__ doneo, double* __restrict__ noko,
+            int* __restrict__ status,
             int* __restrict__ redo, int* __restrict__ list, int* __restrict__ count, double* __restrict__ tries,
             const double* __restrict__ Jin, int B, double rtol, double atol, int max_steps,
             const double* __restrict__ x0d, const double* __restrict__ Td, const double* __restrict__ Pd,
             const double* __restrict__ Jd, double* __restrict__ xd, int adapt, const double* __restrict__ dtd,
             const double* __restrict__ hd, double* __restrict__ hdo, const double* __restrict__ lm,
-            const double* __restrict__ lmd, const double* __restrict__ rcd) {
+            const double* __restrict__ lmd, const double* __restrict__ rcd, int D) {
   extern __shared__ double lds[];
   const int lane = (int)(threadIdx.x & (WAVE - 1)), wv = (int)(threadIdx.x / WAVE);
   double* ws = lds + wv * (L_SIZE + J_SIZE);
@@ -477,11 +481,14 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
   int q = (int)blockIdx.x * CPW + wv;
   double c[NT];
   LeadRow a21;
-  while (q < B) {
-    const int cell = q;
+  const int pairs = B * D;
+  while (q < pairs) {
+    const int dir = q / B, cell = q - dir * B;
+    const bool lead = dir == 0;  // direction 0's pair writes the call's results and counts
+    const size_t dc = (size_t)dir * B + cell;  // the pair's row in the per-direction arrays
     const double Tc = Ta[cell], dt = dta[cell], nd = Pa[cell] / (KBOLTZ * Tc);
     const double* xr = x0 + (size_t)cell * NS;
-    const double* xdr = x0d + (size_t)cell * NS;
+    const double* xdr = x0d + dc * NS;
     const gptr<int> perm = opaque(net)->perm;
     double Nl = lane < K0 ? xr[perm[lane]] : 0.0, Nt = lane < NT ? xr[perm[K0 + lane]] : 0.0;
     double Ndl = lane < K0 ? xdr[perm[lane]] : 0.0, Ndt = lane < NT ? xdr[perm[K0 + lane]] : 0.0;
@@ -498,26 +505,26 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
                       isfinite(dt) && dt >= 0.0;
     const int st = __builtin_amdgcn_readfirstlane(
         Tc < TMIN ? REFUSED_COLD : (Tc > TMAX ? REFUSED_HOT : (fine ? 0 : REFUSED_BAD_INPUT)));
-    if (lane == 0) status[cell] = st;
+    if (lane == 0 && lead) status[cell] = st;
     // nd = P / (k_B T): its tangent
-    const double Tdc = Td ? Td[cell] : 0.0, Pdc = Pd ? Pd[cell] : 0.0;
+    const double Tdc = Td ? Td[dc] : 0.0, Pdc = Pd ? Pd[dc] : 0.0;
     const double ndd = nd * (Pdc / Pa[cell] - Tdc / Tc);
     if (st == 0) {
       cell_rates(ws, rk, Tc, nd, net, Jc, lane, lmc);
-      rates_tangent(ts, rkd, Tc, Tdc, nd, ndd, net, Jc, (NJ && Jd) ? Jd + (size_t)cell * NJ : nullptr, lmc,
-                    lmd ? lmd + (size_t)cell * NR : nullptr, rcd, lane);
+      rates_tangent(ts, rkd, Tc, Tdc, nd, ndd, net, Jc, (NJ && Jd) ? Jd + dc * NJ : nullptr, lmc,
+                    lmd ? lmd + dc * NR : nullptr, rcd ? rcd + (size_t)dir * NR * 9 : nullptr, lane);
     }
-    const double hin = hio[cell];
+    const double hin = hin_a[cell];
     const double h0 = isfinite(hin) ? hin : dt;
-    if (lane == 0 && !isfinite(hin)) atomicAdd(count + COUNT_NONFINITE_H, 1);
+    if (lane == 0 && lead && !isfinite(hin)) atomicAdd(count + COUNT_NONFINITE_H, 1);
     Step s = {fmax(fmin(h0, dt), H_MIN), 0.0, 0.0, 0.0, 0.0, false};
     // the adaptive contract's tangents of the step control (wave-uniform): of s.h, s.t, s.pen, s.pha; the first step
     // min(h, dt) clamped at H_MIN takes the tangent of the side it takes, half of each at a tie
     const bool ad = adapt != 0;
-    const double dtdc = (ad && dtd) ? dtd[cell] : 0.0;
+    const double dtdc = (ad && dtd) ? dtd[dc] : 0.0;
     double hdot = 0.0, tdot = 0.0, pendot = 0.0, phadot = 0.0;
     if (ad) {
-      const double h0d = isfinite(hin) ? (hd ? hd[cell] : 0.0) : dtdc;
+      const double h0d = isfinite(hin) ? (hd ? hd[dc] : 0.0) : dtdc;
       const double md = h0 < dt ? h0d : (h0 > dt ? dtdc : 0.5 * (h0d + dtdc)), m = fmin(h0, dt);
       hdot = m > H_MIN ? md : (m < H_MIN ? 0.0 : 0.5 * md);  // the first step max(m, H_MIN)
     }
@@ -614,7 +621,7 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
     }
     const bool fin = st == 0 && s.t >= dt * CALL_END;
     double* xo = xout + (size_t)cell * NS;
-    double* xdo = xd + (size_t)cell * NS;
+    double* xdo = xd + dc * NS;
     int code = 0;
     if (st == 0)
       code = __builtin_amdgcn_readfirstlane(guard_wave(Nl, Nt, xr, xo, fin, s.nok, atol, net, lane));
@@ -625,9 +632,9 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
     } else {
       for (int i = lane; i < NS; i += WAVE) xdo[i] = DNAN;
     }
-    if (lane == 0) {
-      hio[cell] = s.h;
-      if (hdo) hdo[cell] = !ad ? 0.0 : (st == 0 && code == 0 ? hdot : DNAN);
+    if (lane == 0 && hdo) hdo[dc] = !ad ? 0.0 : (st == 0 && code == 0 ? hdot : DNAN);
+    if (lane == 0 && lead) {
+      hout[cell] = s.h;
       doneo[cell] = fin ? 1 : 0;
       noko[cell] = s.nok;
       if (tries) tries[cell] = (double)ntry;

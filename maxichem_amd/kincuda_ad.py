@@ -34,6 +34,7 @@ from .kincuda import CudaSolver
 __all__ = ["ADSolver", "Cotangents", "JvpResult", "VjpResult"]
 DMAX = 64  # directions of one jvp call (the reference's bound on the leading axis)
 ARRHENIUS_COLUMNS = 9  # ln A, B, C of k0, of the second k0 and of k_inf: the differentiated columns of rc
+COLUMN_BYTES = 1 << 31  # device memory for one launch's unit tangents and outputs in vjp
 
 
 class JvpResult(NamedTuple):
@@ -139,7 +140,8 @@ class ADSolver(CudaSolver):
         """The call and its tangent. Everything after dh is keyword only.
 
         dx, dT, dP, ddt, dh: the inputs' tangents in their shapes, or with a leading axis of D <= 64 directions on
-        every one; dJ [B, branches] (or [D, B, branches]) with photolysis. adapt=True (the reference's default)
+        every one, which then share one launch (each direction repeats the call with the same bits); dJ [B, branches]
+        (or [D, B, branches]) with photolysis. adapt=True (the reference's default)
         differentiates the step control too: every try's tangent, the error's and the step factor's, so ddt and dh
         enter and the result's h has a tangent; adapt=False holds the accepted steps fixed (ddt and dh do not enter,
         the result's dh is zero). The result's x, h, done and steps are CudaSolver.launch's; dx, dh the tangents of x
@@ -175,31 +177,25 @@ class ADSolver(CudaSolver):
             lm, net, table = self._rates(B, lm, rc)
             jx = check_photolysis(J, B, self.nj, self.dev)
             dJ = dJ if self.nj else None
-            tx = dx if not one else dx[None]
-            tT, tP, tdt, th = (None if a is None else (a if not one else a[None]) for a in (dT, dP, ddt, dh))
-            tJ = None if dJ is None else (dJ if not one else dJ[None])
-            tlm, trc = (None if a is None else (a if not one else a[None]) for a in (dlm, drc))
+            tx = (dx if not one else dx[None]).contiguous()
+            tT, tP, tdt, th, tJ, tlm, trc = (None if a is None else (a if not one else a[None]).contiguous()
+                                             for a in (dT, dP, ddt, dh, dJ, dlm, drc))  # fmt: skip
             xd = torch.empty(D, B, self.ns, dtype=torch.float64, device=self.dev)
             hd = torch.empty(D, B, dtype=torch.float64, device=self.dev)
-            result = None
-
-            def at(a, d):
-                return None if a is None else a[d].contiguous()
-
-            for d in range(D):  # each direction repeats the call, with the same bits
-                result = self._jvp_once(x, T, P, dt, h, jx, rtol, atol, max_steps, B, tx[d].contiguous(), at(tT, d),
-                                        at(tP, d), at(tJ, d), xd[d], adapt, at(tdt, d), at(th, d), hd[d], lm,
-                                        at(tlm, d), at(trc, d), net)  # fmt: skip
+            xo, ho, done, nok, redo = self._jvp_launch(x, T, P, dt, h, jx, rtol, atol, max_steps, B, D, tx, tT, tP, tJ,
+                                                       xd, adapt, tdt, th, hd, lm, tlm, trc, net)  # fmt: skip
             del table  # kept to here: the launches above read it
-            xo, ho, done, nok, redo = result
             self.total += self.count
             self._refuse(x, T, P, dt, jx, self._status)
         self.flag = redo
         return JvpResult(xo, ho, done.bool(), nok, xd[0] if one else xd, hd[0] if one else hd)
 
-    def _jvp_once(self, x, T, P, dt, h, jx, rtol, atol, max_steps, B, dx, dT, dP, dJ, xd, adapt=False, ddt=None,
-                  dh=None, hd=None, lm=None, dlm=None, drc=None, net=None):  # fmt: skip
-        xo, ho = torch.empty_like(x), h.clone()
+    def _jvp_launch(self, x, T, P, dt, h, jx, rtol, atol, max_steps, B, D, dx, dT, dP, dJ, xd, adapt=False, ddt=None,
+                    dh=None, hd=None, lm=None, dlm=None, drc=None, net=None):  # fmt: skip
+        """One launch of the tangent kernel over the D directions' (direction, cell) pairs (each tangent [D, ...]
+        contiguous, or None), then the recovery of the listed cells with each direction's tangent. Returns the call's
+        x, h, done, steps and flags; the tangents go to xd [D, B, S] and hd [D, B]."""
+        xo, ho = torch.empty_like(x), torch.empty_like(h)
         done, redo, lst, status = (torch.empty(B, dtype=torch.int32, device=self.dev) for _ in range(4))
         nok = torch.empty(B, dtype=torch.float64, device=self.dev)
         self.tries = torch.zeros(B, dtype=torch.float64, device=self.dev)
@@ -207,21 +203,39 @@ class ADSolver(CudaSolver):
         p = _hip.ptr
         stream = torch.cuda.current_stream(self.dev).cuda_stream
         net = self._net if net is None else net
-        args = [p(net), p(x), p(T), p(P), p(dt), p(ho), p(xo), p(done), p(nok), p(status), p(redo), p(lst),
+        args = [p(net), p(x), p(T), p(P), p(dt), p(h), p(ho), p(xo), p(done), p(nok), p(status), p(redo), p(lst),
                 p(self.count), p(self.tries), p(jx), ctypes.c_int(B), ctypes.c_double(rtol), ctypes.c_double(atol),
                 ctypes.c_int(max_steps), p(dx), p(dT), p(dP), p(dJ), p(xd), ctypes.c_int(int(bool(adapt))), p(ddt),
-                p(dh), p(hd), p(lm), p(dlm), p(drc)]  # fmt: skip
+                p(dh), p(hd), p(lm), p(dlm), p(drc), ctypes.c_int(D)]  # fmt: skip
         if B:
-            _hip.launch(self._jvp, max(1, min(B, self.max_blocks_jvp)), WAVE, self.lds_jvp, stream, args)
-        # the cells the guard failed: the recovery's result with its tangent
-        rec_args = [p(net), p(x), p(T), p(P), p(dt), p(h), p(xo), p(ho), p(done), p(nok), p(redo), p(lst),
-                    p(self.count), p(self.tries), p(jx), ctypes.c_double(rtol), ctypes.c_double(atol),
-                    ctypes.c_int(REC_STEPS * max_steps), p(lm), p(dx), p(dT), p(dP), p(dJ), p(xd),
-                    ctypes.c_int(int(bool(adapt))), p(ddt), p(dh), p(hd), p(dlm), p(drc)]  # fmt: skip
-        _hip.launch(self._jvp_recover, max(1, min(B, self.rec_blocks_jvp)), WAVE * _kernels.REC_WAVES,
-                    self.lds_jvp_rec, stream, rec_args)  # fmt: skip
+            _hip.launch(self._jvp, max(1, min(B * D, self.max_blocks_jvp)), WAVE, self.lds_jvp, stream, args)
+        # the cells the guard failed: the recovery's result with each direction's tangent. Direction 0's launch keeps
+        # the counts and the tries; the others run on a copy of the counts as the tangent kernel left them (the list's
+        # length), write the same results again and leave the tries alone
+        listed = self.count.clone()
+
+        def row(a, d):
+            return None if a is None else a[d]
+
+        for d in range(D):
+            count = self.count if d == 0 else listed.clone()
+            rec_args = [p(net), p(x), p(T), p(P), p(dt), p(h), p(xo), p(ho), p(done), p(nok), p(redo), p(lst), p(count),
+                        p(self.tries if d == 0 else None), p(jx), ctypes.c_double(rtol), ctypes.c_double(atol),
+                        ctypes.c_int(REC_STEPS * max_steps), p(lm), p(dx[d]), p(row(dT, d)), p(row(dP, d)),
+                        p(row(dJ, d)), p(xd[d]), ctypes.c_int(int(bool(adapt))), p(row(ddt, d)), p(row(dh, d)),
+                        p(hd[d]), p(row(dlm, d)), p(row(drc, d))]  # fmt: skip
+            _hip.launch(self._jvp_recover, max(1, min(B, self.rec_blocks_jvp)), WAVE * _kernels.REC_WAVES,
+                        self.lds_jvp_rec, stream, rec_args)  # fmt: skip
         self._status = status
         return xo, ho, done, nok, redo
+
+    def _column_chunks(self, columns, B: int):
+        """The vjp's columns in launches of at most DMAX directions, each launch's unit tangents within
+        COLUMN_BYTES."""
+        per = 8 * B * (2 * self.ns + 4 + self.nj + self.nr)  # one direction's tangents and outputs, at most
+        size = max(1, min(DMAX, COLUMN_BYTES // max(per, 1)))
+        for i in range(0, len(columns), size):
+            yield columns[i : i + size]
 
     def vjp(self, x, T, P, dt, h, gx, *, gh=None, lm=None, rc=None, adapt: bool = True, rtol: float = DEFAULT_RTOL,
             atol: float = ATOL, max_steps: int = DEFAULT_MAX_STEPS, J=None, rates: bool = True) -> VjpResult:  # fmt: skip
@@ -229,13 +243,14 @@ class ADSolver(CudaSolver):
         either contract as jvp (the reference's ADSolver.vjp, maxichem/kincuda_ad.py:457-556). Everything after gx is
         keyword only; lm [B, R] and rc [R, W] as jvp.
 
-        Built from the tangent by columns, not by a reverse pass: one tangent call per input coordinate (each of x's
-        S entries, T, P, each photolysis branch, under adapt=True dt and h, and with rates each thermal reaction's
-        multiplier), every cell along the same unit direction at once, and each cell's cotangent gx . dx + gh dh of
-        its column. A photolysis branch's multiplier cotangent is J times J's (kf = J exp(lm)); the rate table's is
-        reduced from the multipliers' by mxc_rc_cotangent (a multiplier and the law enter a reaction's coefficients
-        alike, the law's share 1 - wA of the collision limit). It costs S + 2 + branches (+ 2) tangent calls, and the
-        thermal reactions' count more with rates (rates=False leaves the cotangents of lm and rc None). Under the
+        Built from the tangent by columns, not by a reverse pass: one tangent direction per input coordinate (each
+        of x's S entries, T, P, each photolysis branch, under adapt=True dt and h, and with rates each thermal
+        reaction's multiplier), every cell along the same unit direction, up to 64 directions in one launch, and each
+        cell's cotangent gx . dx + gh dh of its column. A photolysis branch's multiplier cotangent is J times J's
+        (kf = J exp(lm)); the rate table's is reduced from the multipliers' by mxc_rc_cotangent (a multiplier and the
+        law enter a reaction's coefficients alike, the law's share 1 - wA of the collision limit). Its work is S + 2 +
+        branches (+ 2) tangent calls, and the thermal reactions' count more with rates (rates=False leaves the
+        cotangents of lm and rc None); a small batch runs its columns side by side. Under the
         frozen contract the cotangents of dt and h are zero. A cell that still fails the guard (flag 2) has NaN
         cotangents and is left out of the rate table's sum (fallback["still_failed"] counts them)."""
         self.flag = None
@@ -250,49 +265,46 @@ class ADSolver(CudaSolver):
             lm, net, table = self._rates(B, lm, rc)
             jx = check_photolysis(J, B, self.nj, self.dev)
             zero = torch.zeros(B, dtype=torch.float64, device=self.dev)
-            col = torch.empty(B, self.ns, dtype=torch.float64, device=self.dev)
-            colh = torch.empty(B, dtype=torch.float64, device=self.dev)
             g = Cotangents(x=torch.empty_like(x), T=torch.empty_like(T), P=torch.empty_like(T), dt=zero.clone(),
                            h=zero.clone(),
                            lm=torch.empty(B, self.nr, dtype=torch.float64, device=self.dev) if rates else None,
                            rc=torch.empty(self.nr, ARRHENIUS_COLUMNS, dtype=torch.float64, device=self.dev)
                            if rates else None,
                            J=torch.empty(B, self.nj, dtype=torch.float64, device=self.dev) if self.nj else None)  # fmt: skip
-            unit = torch.zeros(B, self.ns, dtype=torch.float64, device=self.dev)
-            unit_j = torch.zeros(B, self.nj, dtype=torch.float64, device=self.dev) if self.nj else None
-            ones = torch.ones(B, dtype=torch.float64, device=self.dev)
-            result = None
-
-            def column(dx, dT=None, dP=None, dJ=None, ddt=None, dh=None, dlm=None):
-                nonlocal result
-                result = self._jvp_once(x, T, P, dt, h, jx, rtol, atol, max_steps, B, dx, dT, dP, dJ, col, adapt, ddt,
-                                        dh, colh, lm, dlm, None, net)  # fmt: skip
-                out = (gx * col).sum(1)
-                return out if gh is None or not adapt else out + gh * colh
-
-            for s in range(self.ns):  # x's coordinates
-                unit.zero_()
-                unit[:, s] = 1.0
-                g.x[:, s] = column(unit)
-            unit.zero_()
-            g.T.copy_(column(unit, dT=ones))
-            g.P.copy_(column(unit, dP=ones))
-            for b in range(self.nj):  # the photolysis branches
-                unit_j.zero_()
-                unit_j[:, b] = 1.0
-                g.J[:, b] = column(unit, dJ=unit_j)
+            # the input coordinates, each a unit direction for every cell at once: (what, index, where its
+            # cotangent goes)
+            columns = [("x", s, g.x[:, s]) for s in range(self.ns)] + [("T", 0, g.T), ("P", 0, g.P)]
+            columns += [("J", b, g.J[:, b]) for b in range(self.nj)]
             if adapt:  # dt and h enter under the adaptive contract
-                g.dt.copy_(column(unit, ddt=ones))
-                g.h.copy_(column(unit, dh=ones))
-            if rates:
-                nrt = self.tab.nrt
-                unit_r = torch.zeros(B, self.nr, dtype=torch.float64, device=self.dev)
-                for r in range(nrt):  # the thermal reactions' multipliers
-                    unit_r.zero_()
-                    unit_r[:, r] = 1.0
-                    g.lm[:, r] = column(unit, dlm=unit_r)
-                if self.nj:  # a branch's kf = J exp(lm): d/dlm = J d/dJ
-                    g.lm[:, nrt:] = jx * g.J
+                columns += [("dt", 0, g.dt), ("h", 0, g.h)]
+            if rates:  # the thermal reactions' multipliers
+                columns += [("lm", r, g.lm[:, r]) for r in range(self.tab.nrt)]
+            result = None
+            for chunk in self._column_chunks(columns, B):
+                D = len(chunk)
+                kinds = {k for k, _, _ in chunk}
+                f64 = {"dtype": torch.float64, "device": self.dev}
+                dx = torch.zeros(D, B, self.ns, **f64)
+                dT, dP, ddt, dh = (torch.zeros(D, B, **f64) if k in kinds else None for k in ("T", "P", "dt", "h"))
+                dJ = torch.zeros(D, B, self.nj, **f64) if "J" in kinds else None
+                dlm = torch.zeros(D, B, self.nr, **f64) if "lm" in kinds else None
+                unit = {"x": dx, "T": dT, "P": dP, "dt": ddt, "h": dh, "J": dJ, "lm": dlm}
+                for d, (k, i, _) in enumerate(chunk):
+                    if k in ("x", "J", "lm"):
+                        unit[k][d, :, i] = 1.0
+                    else:
+                        unit[k][d] = 1.0
+                xd = torch.empty(D, B, self.ns, **f64)
+                hd = torch.empty(D, B, **f64)
+                result = self._jvp_launch(x, T, P, dt, h, jx, rtol, atol, max_steps, B, D, dx, dT, dP, dJ, xd, adapt,
+                                          ddt, dh, hd, lm, dlm, None, net)  # fmt: skip
+                out = torch.einsum("dbs,bs->db", xd, gx)
+                if gh is not None and adapt:
+                    out += hd * gh
+                for d, (_, _, dest) in enumerate(chunk):
+                    dest.copy_(out[d])
+            if rates and self.nj:  # a branch's kf = J exp(lm): d/dlm = J d/dJ
+                g.lm[:, self.tab.nrt:] = jx * g.J
             xo, ho, done, nok, redo = result
             if rates:
                 skip = (redo == 2).to(torch.int32)  # a cell that still fails has no tangent: left out of the sum

@@ -34,7 +34,7 @@ def write(folder: str | Path, *, tab: _tables.Tables, source: str, code: bytes, 
     (folder / "source").mkdir(exist_ok=True)
     (folder / "source" / "maxichem_amd.hip").write_text(source)
     (folder / CODE_FILE).write_bytes(code)
-    lds_call, lds_rec = _kernels.lds_bytes(tab, cpw, precision == "mixed", rates_global)
+    lds_call, lds_rec = _kernels.lds_bytes(tab, cpw, precision == "mixed", rates_global, wps)
     if len(species) != tab.ns or len(elements) != tab.nel:
         raise ValueError(f"{len(species)} species and {len(elements)} elements named, the tables have {tab.ns} and "
                          f"{tab.nel}")  # fmt: skip

@@ -51,8 +51,8 @@ __device__ __forceinline__ bool stages(T (&c)[NT], LeadRow& a21, double Nl, doub
                                        double& K4t, StageKeep* keep = nullptr, DT* dtr_keep = nullptr) {
   const double igh = 1.0 / (GAMMA * ha), ih = 1.0 / ha;
   double Fl = 0.0, Ft = 0.0, ul = 0.0, ut = 0.0, u0l = 0.0, u0t = 0.0;
-  const double ntot = rhs<true>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
-  jacobian(c, a21, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
+  const double ntot = rhs<true, WORDS_LDS != 0>(Nl, Nt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
+  jacobian<WORDS_LDS != 0>(c, a21, Nl, Nt, u0l, u0t, ntot, ul, ut, net, ws, lane);
   add_diagonal(c, igh, ws, lane);
   double dl;
   T dtr;
@@ -74,7 +74,7 @@ __device__ __forceinline__ bool stages(T (&c)[NT], LeadRow& a21, double Nl, doub
   sk = vsum(y1l[0], y1t[0], lane) * c0;
   const double K2l = y1l[0] + sk * Ul, K2t = y1t[0] + sk * Ut;
   double Yl = Nl + 2.0 * K1l, Yt = Nt + 2.0 * K1t;
-  rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
+  rhs<false, WORDS_LDS != 0>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
   y1l[0] = fma(ih, K1l - K2l, Fl);
   y1t[0] = fma(ih, K1t - K2t, Ft);
   MXC_SOLVE(y1l, y1t);
@@ -83,7 +83,7 @@ __device__ __forceinline__ bool stages(T (&c)[NT], LeadRow& a21, double Nl, doub
   K3t = y1t[0] + sk * Ut;
   Yl = Nl + 2.0 * K1l + K3l;
   Yt = Nt + 2.0 * K1t + K3t;
-  rhs<false>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
+  rhs<false, WORDS_LDS != 0>(Yl, Yt, Fl, Ft, u0l, u0t, net, nd, ws, rk, lane);
   y1l[0] = fma(ih, fma(-RODAS_C43, K3l, K1l - K2l), Fl);
   y1t[0] = fma(ih, fma(-RODAS_C43, K3t, K1t - K2t), Ft);
   MXC_SOLVE(y1l, y1t);
@@ -114,6 +114,17 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW) __attribute__((amdgpu_wa
 #endif
   const int waves = (int)gridDim.x * CPW;
   int q = (int)blockIdx.x * CPW + wv;
+#if WORDS_LDS
+  {  // the stream words into the workgroup's LDS, after its cells' regions
+    const NetP nt = opaque(net);
+    u32* lw = (u32*)(lds + CPW * L_SIZE);
+    for (int i = (int)threadIdx.x; i < SV_ROUNDS * WAVE; i += WAVE * CPW) lw[W_SV + i] = nt->svw[i];
+    for (int i = (int)threadIdx.x; i < DIAG_ROUNDS * WAVE; i += WAVE * CPW) lw[W_DIAG + i] = nt->diagw[i];
+    for (int i = (int)threadIdx.x; i < LEAD_ROUNDS * WAVE; i += WAVE * CPW) lw[W_LEAD + i] = nt->leadw[i];
+    for (int i = (int)threadIdx.x; i < WORDS_TOTAL - W_CH; i += WAVE * CPW) lw[W_CH + i] = nt->chw[i];
+    __syncthreads();
+  }
+#endif
   double c[NT];  // the trailing block of M0 and its factors (in the mixed form, once a cell's FP32 refinement failed)
   LeadRow a21;  // the trailing row's lead columns of M0, then of L
   while (q < B) {

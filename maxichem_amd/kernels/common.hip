@@ -286,9 +286,9 @@ __device__ __forceinline__ void cell_rates(double* ws, double* rk, double T, dou
 // destination in dst. The words of STREAM_BATCH rounds are loaded together, ahead of their terms, so that their
 // latency is paid once per batch; rounds past the end take the padding word pad (a zero term, no end). (Loading each
 // batch, or a stream's first batch, one step ahead was measured slower: it holds 36 more registers.)
-template <unsigned IDX_MASK, int DBL, int END, int DEST, unsigned DEST_MASK>
-__device__ __forceinline__ void stream_sum(gptr<u32> __restrict__ words, int rounds, u32 pad, const double* src,
-                                           double* dst, int lane) {
+template <unsigned IDX_MASK, int DBL, int END, int DEST, unsigned DEST_MASK, typename WP>
+__device__ __forceinline__ void stream_sum(WP __restrict__ words, int rounds, u32 pad, const double* src, double* dst,
+                                           int lane) {
   double acc = 0.0;
 #pragma unroll 1
   for (int q0 = 0; q0 < rounds; q0 += STREAM_BATCH) {
@@ -310,14 +310,24 @@ __device__ __forceinline__ void stream_sum(gptr<u32> __restrict__ words, int rou
 }
 
 // One Jacobian stream into the cell's LDS: entries of -Jn at their offsets.
-__device__ __forceinline__ void jac_stream(gptr<u32> __restrict__ words, int rounds, const double* DV, double* ws,
-                                           int lane) {
+template <typename WP>
+__device__ __forceinline__ void jac_stream(WP __restrict__ words, int rounds, const double* DV, double* ws, int lane) {
   stream_sum<T_IDX_MASK, T_DBL, T_END, T_DEST, T_DEST_MASK>(words, rounds, (u32)NDV, DV, ws, lane);
+}
+
+// The stream words in the workgroup's LDS (WORDS_LDS: the call kernel copies them there at its start, after its
+// cells' regions), in the order S v, the Jacobian's diagonal, lead and chunk streams: read with the LDS's latency
+// instead of the caches'.
+constexpr int W_SV = 0, W_DIAG = SV_ROUNDS * WAVE, W_LEAD = W_DIAG + DIAG_ROUNDS * WAVE;
+constexpr int W_CH = W_LEAD + LEAD_ROUNDS * WAVE;
+__device__ __forceinline__ const u32* lds_words(const double* ws) {  // ws: this wavefront's cell region
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+  return (const u32*)(ws + (CPW - wv) * L_SIZE);
 }
 
 // F = (n_tot / nd) S v(n) at the state Y (lead / trailing parts), u0 = S v / nd; with JAC also the partial
 // derivatives dv/dn of every live reaction into DV. Returns n_tot = sum Y.
-template <bool JAC>
+template <bool JAC, bool LW = false>
 __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& Ft, double& u0l, double& u0t,
                                       NetP net, double nd, double* ws, const double* rk, int lane) {
   const double ntot = wave_sum((lane < K0 ? Yl : 0.0) + (lane < NT ? Yt : 0.0));
@@ -376,7 +386,11 @@ __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& 
   if (lane == 0) V[NR] = 0.0;  // the S v stream's padding term
   wave_sync();
   double* part = ws + L_PART;
-  stream_sum<SV_R_MASK, SV_DBL, SV_END, SV_PIECE, SV_PIECE_MASK>(svw, SV_ROUNDS, (u32)NR, V, part, lane);
+  if constexpr (LW)
+    stream_sum<SV_R_MASK, SV_DBL, SV_END, SV_PIECE, SV_PIECE_MASK>(lds_words(ws) + W_SV, SV_ROUNDS, (u32)NR, V, part,
+                                                                  lane);
+  else
+    stream_sum<SV_R_MASK, SV_DBL, SV_END, SV_PIECE, SV_PIECE_MASK>(svw, SV_ROUNDS, (u32)NR, V, part, lane);
   wave_sync();
   if (lane < K0) {
     double s = 0.0;
@@ -398,7 +412,7 @@ __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& 
 // -Jn into LEAD (LDS), the trailing rows' lead columns a21 and the trailing block c (registers), from the dv terms
 // that rhs<true> left in DV; then u = w = S v / nd - Jn N / n_tot, the constant-pressure part of the exact Jacobian
 // (lead / trailing parts).
-template <typename T>
+template <bool LW = false, typename T>
 __device__ __forceinline__ void jacobian(T (&c)[NT], LeadRow& a21, double Nl, double Nt, double u0l, double u0t,
                                          double ntot, double& ul, double& ut, NetP net, double* ws, int lane) {
   double* DV = ws + L_DV;
@@ -410,8 +424,13 @@ __device__ __forceinline__ void jacobian(T (&c)[NT], LeadRow& a21, double Nl, do
   if (lane < K0) xv[lane] = Nl;
   if (lane < NT) xv[K0 + lane] = Nt;
   wave_sync();
-  jac_stream(opaque(net)->diagw, DIAG_ROUNDS, DV, ws, lane);
-  jac_stream(opaque(net)->leadw, LEAD_ROUNDS, DV, ws, lane);
+  if constexpr (LW) {
+    jac_stream(lds_words(ws) + W_DIAG, DIAG_ROUNDS, DV, ws, lane);
+    jac_stream(lds_words(ws) + W_LEAD, LEAD_ROUNDS, DV, ws, lane);
+  } else {
+    jac_stream(opaque(net)->diagw, DIAG_ROUNDS, DV, ws, lane);
+    jac_stream(opaque(net)->leadw, LEAD_ROUNDS, DV, ws, lane);
+  }
   wave_sync();
   // -Jn N in FP64, from the entries before the trailing block takes them in T: the lead rows through the exchange
   // vector, then the trailing rows chunk by chunk (their lead columns, then the trailing ones), N_j from lane j
@@ -454,7 +473,10 @@ __device__ __forceinline__ void jacobian(T (&c)[NT], LeadRow& a21, double Nl, do
       if (ch >= NLCH && lane / CHUNK == ch - NLCH) stage[(lane % CHUNK) * NT + lane] = cd;
     }
     wave_sync();
-    jac_stream(opaque(net)->chw + CH_OFF[ch] * WAVE, CH_ROUNDS[ch], DV, ws, lane);
+    if constexpr (LW)
+      jac_stream(lds_words(ws) + W_CH + CH_OFF[ch] * WAVE, CH_ROUNDS[ch], DV, ws, lane);
+    else
+      jac_stream(opaque(net)->chw + CH_OFF[ch] * WAVE, CH_ROUNDS[ch], DV, ws, lane);
     wave_sync();
     // N_j for -Jn N as wave-uniform values: with few lead rows from the exchange vector (same-address LDS reads,
     // kept in their chunk: hoisted ahead, they would hold a register pair each), else by v_readlane (with the lead

@@ -108,7 +108,8 @@ class CudaSolver(Solver):
         dev = self.dev.index
         info = _hip.device_info(dev)
         mixed = self.precision == "mixed"
-        self.lds_call, self.lds_rec = _kernels.lds_bytes(tab, self.cpw, mixed, bool(self.rates_global))
+        self.lds_call, self.lds_rec = _kernels.lds_bytes(tab, self.cpw, mixed, bool(self.rates_global), self.wps,
+                                                         self._EXTRA_KERNELS)  # fmt: skip
         for what, need in (("call", self.lds_call), ("recovery", self.lds_rec)):
             if need > info["lds_per_block"]:
                 raise ValueError(
@@ -142,6 +143,8 @@ class CudaSolver(Solver):
             "ns": tab.ns, "nr": tab.nr, "nj": tab.nj, "k0": tab.k0, "nt": tab.nt, "ndv": tab.ndv,
             "jac_terms": tab.jac_terms, "sv_terms": tab.sv_terms, "lds_call": self.lds_call, "lds_rec": self.lds_rec,
             "regs": _hip.attribute(self._call, _hip.ATTR_REGISTERS),
+            "words_in_lds": not self._EXTRA_KERNELS and _kernels.words_in_lds(
+                tab, self.cpw, self.precision == "mixed", self.wps, bool(self.rates_global)),
             "private_bytes": _hip.attribute(self._call, _hip.ATTR_PRIVATE_BYTES),
             "cells_per_cu": self.max_blocks // cus * self.cpw, "max_blocks": self.max_blocks,
             "rec_blocks": self.rec_blocks, "compute_units": cus, "drift": self.drift, "tables": tab.digest(),

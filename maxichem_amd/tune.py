@@ -27,13 +27,18 @@ CU_WAVES = 4  # call wavefronts per compute unit at one per SIMD
 
 
 def default_variant(tab, mixed: bool = False) -> tuple[int, int, int, int]:
-    """The variant without a stored choice: two cells per workgroup when that keeps as many cells per compute unit
-    as one (on MI355X the call kernel then runs about 2 % faster, with the same bits; single-wavefront workgroups
-    leave the SIMDs less evenly loaded), else one."""
+    """The variant without a stored choice, with the same bits as every other: four cells per workgroup when the
+    stream words then fit in the workgroup's LDS at as many cells per compute unit (on MI355X 2.6 % faster on NCHO:
+    the words come with the LDS's latency instead of the caches'); else two when that keeps as many cells per
+    compute unit as one (about 2 % faster: single-wavefront workgroups leave the SIMDs less evenly loaded); else
+    one."""
 
     def cells(cpw: int) -> int:
         return min(CU_WAVES, CU_LDS // _kernels.lds_bytes(tab, cpw, mixed)[0] * cpw)
 
+    best = max(cells(c) for c in (1, 2, 4))
+    if _kernels.words_in_lds(tab, 4, mixed) and cells(4) >= best:
+        return (4, _kernels.DEFAULT_BATCH, 1, 0)
     return (2 if cells(2) >= cells(1) else 1, _kernels.DEFAULT_BATCH, 1, 0)
 
 

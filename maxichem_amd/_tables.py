@@ -174,7 +174,8 @@ class Tables:
         """Offsets (doubles) in a cell's LDS region: LEAD [K0][NS], n [NS + 1], dv [NDV + 1], then a
         union of what the rhs uses (v [NR + 1], pieces [NPART]) with what the Jacobian and the factorization use after
         it (the trailing diagonal [NT], staging [CHUNK][NT], column-major so that a lane's writes and reads of its row
-        fall in distinct banks); G / RT [NS + 1] sits at its start while the rates are computed. mixed: the mixed-precision form's exchange vectors [2][NS + 1] follow; then the rate coefficients
+        fall in distinct banks); G / RT [NS + 1] sits at its start while the rates are computed; a slot per lane
+        (TRASH [WAVE]) where the streams write the sums of the words that end no entry. mixed: the mixed-precision form's exchange vectors [2][NS + 1] follow; then the rate coefficients
         KF, KR [NR] (kept for the call) unless rates_lds is False (they are then in global scratch). Everything the
         Jacobian streams address lies before the optional parts, so the streams hold for every layout."""
         nvec = self.k0 * self.ns  # the trailing rows' lead columns (A21) are held in registers, one row per lane
@@ -185,6 +186,8 @@ class Tables:
         diag = union
         stage = diag + self.nt
         size = max(part + self.npart, stage + self.nt * CHUNK, union + self.ns + 1)
+        trash = size  # a slot per lane for the stream words that end no entry (written, never read)
+        size += WAVE
         xv = size  # the mixed-precision form's exchange vectors [2][NS + 1]
         if mixed:
             size += 2 * (self.ns + 1)
@@ -194,7 +197,7 @@ class Tables:
             size = kr + self.nr
         size += size & 1  # keep the next cell's region 16-byte aligned
         return {"NVEC": nvec, "KF": kf, "KR": kr, "DV": dv, "UNION": union, "V": v, "PART": part,
-                "DIAG": diag, "STAGE": stage, "XV": xv, "SIZE": size}
+                "DIAG": diag, "STAGE": stage, "TRASH": trash, "XV": xv, "SIZE": size}
 
 
 def build(net: VulcanNetwork, photo: bool = False) -> Tables:

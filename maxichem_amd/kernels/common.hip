@@ -284,11 +284,13 @@ __device__ __forceinline__ void cell_rates(double* ws, double* rk, double T, dou
 
 // A term stream (the Jacobian's and S v's): each lane sums whole entries from src terms and writes each at its
 // destination in dst. The words of STREAM_BATCH rounds are loaded together, ahead of their terms, so that their
-// latency is paid once per batch; rounds past the end take the padding word pad (a zero term, no end). (Loading each
+// latency is paid once per batch; rounds past the end load the padding word pad (a zero term, no end). (Loading each
 // batch, or a stream's first batch, one step ahead was measured slower: it holds 36 more registers.)
+// Every word stores the running sum, at its entry's destination when it ends one and else in the lane's trash
+// slot, and the sum restarts where an entry ended: no branch and no exec-mask change per word, and the same sums.
 template <unsigned IDX_MASK, int DBL, int END, int DEST, unsigned DEST_MASK, typename WP>
 __device__ __forceinline__ void stream_sum(WP __restrict__ words, int rounds, u32 pad, const double* src, double* dst,
-                                           int lane) {
+                                           double* trash, int lane) {
   double acc = 0.0;
 #pragma unroll 1
   for (int q0 = 0; q0 < rounds; q0 += STREAM_BATCH) {
@@ -300,11 +302,14 @@ __device__ __forceinline__ void stream_sum(WP __restrict__ words, int rounds, u3
     for (int i = 0; i < STREAM_BATCH; i++) d[i] = src[w[i] & IDX_MASK];
 #pragma unroll
     for (int i = 0; i < STREAM_BATCH; i++) {
+      // past the end of a stream of one batch, no term: its rounds are known when the kernel is built, so this folds
+      // away (a longer stream's last batch keeps its padding terms, which add zeros: a branch per word there costs
+      // more than the terms)
+      if (rounds <= STREAM_BATCH && q0 + i >= rounds) break;
       acc += term<DBL>(w[i], d[i]);
-      if ((w[i] >> END) & 1u) {
-        dst[(w[i] >> DEST) & DEST_MASK] = acc;
-        acc = 0.0;
-      }
+      const bool end = (w[i] >> END) & 1u;
+      *(end ? dst + ((w[i] >> DEST) & DEST_MASK) : trash) = acc;
+      acc = end ? 0.0 : acc;
     }
   }
 }
@@ -312,7 +317,7 @@ __device__ __forceinline__ void stream_sum(WP __restrict__ words, int rounds, u3
 // One Jacobian stream into the cell's LDS: entries of -Jn at their offsets.
 template <typename WP>
 __device__ __forceinline__ void jac_stream(WP __restrict__ words, int rounds, const double* DV, double* ws, int lane) {
-  stream_sum<T_IDX_MASK, T_DBL, T_END, T_DEST, T_DEST_MASK>(words, rounds, (u32)NDV, DV, ws, lane);
+  stream_sum<T_IDX_MASK, T_DBL, T_END, T_DEST, T_DEST_MASK>(words, rounds, (u32)NDV, DV, ws, ws + L_TRASH + lane, lane);
 }
 
 // The stream words in the workgroup's LDS (WORDS_LDS: the call kernel copies them there at its start, after its
@@ -388,9 +393,10 @@ __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& 
   double* part = ws + L_PART;
   if constexpr (LW)
     stream_sum<SV_R_MASK, SV_DBL, SV_END, SV_PIECE, SV_PIECE_MASK>(lds_words(ws) + W_SV, SV_ROUNDS, (u32)NR, V, part,
-                                                                  lane);
+                                                                  ws + L_TRASH + lane, lane);
   else
-    stream_sum<SV_R_MASK, SV_DBL, SV_END, SV_PIECE, SV_PIECE_MASK>(svw, SV_ROUNDS, (u32)NR, V, part, lane);
+    stream_sum<SV_R_MASK, SV_DBL, SV_END, SV_PIECE, SV_PIECE_MASK>(svw, SV_ROUNDS, (u32)NR, V, part,
+                                                                  ws + L_TRASH + lane, lane);
   wave_sync();
   if (lane < K0) {
     double s = 0.0;

@@ -284,40 +284,43 @@ __device__ __forceinline__ void cell_rates(double* ws, double* rk, double T, dou
 
 // A term stream (the Jacobian's and S v's): each lane sums whole entries from src terms and writes each at its
 // destination in dst. The words of STREAM_BATCH rounds are loaded together, ahead of their terms, so that their
-// latency is paid once per batch; rounds past the end load the padding word pad (a zero term, no end). (Loading each
-// batch, or a stream's first batch, one step ahead was measured slower: it holds 36 more registers.)
-// Every word stores the running sum, at its entry's destination when it ends one and else in the lane's trash
-// slot, and the sum restarts where an entry ended: no branch and no exec-mask change per word, and the same sums.
+// latency is paid once per batch: whole batches in a loop, then the last rounds as a batch of their own (the rounds
+// are known when the kernel is built, so no padding terms are taken). Every word stores the running sum, at its
+// entry's destination when it ends one and else in the lane's trash slot, and the sum restarts where an entry ended:
+// no branch and no exec-mask change per word. (Loading each batch, or a stream's first batch, one step ahead was
+// measured slower: it holds 36 more registers.)
 template <unsigned IDX_MASK, int DBL, int END, int DEST, unsigned DEST_MASK, typename WP>
-__device__ __forceinline__ void stream_sum(WP __restrict__ words, int rounds, u32 pad, const double* src, double* dst,
+__device__ __forceinline__ void stream_sum(WP __restrict__ words, int rounds, const double* src, double* dst,
                                            double* trash, int lane) {
   double acc = 0.0;
-#pragma unroll 1
-  for (int q0 = 0; q0 < rounds; q0 += STREAM_BATCH) {
+  auto batch = [&](int q0, int n) {  // rounds q0 .. q0 + n - 1, n <= STREAM_BATCH
     u32 w[STREAM_BATCH];
     double d[STREAM_BATCH];
 #pragma unroll
-    for (int i = 0; i < STREAM_BATCH; i++) w[i] = q0 + i < rounds ? words[(q0 + i) * WAVE + lane] : pad;
+    for (int i = 0; i < STREAM_BATCH; i++)
+      if (i < n) w[i] = words[(q0 + i) * WAVE + lane];
 #pragma unroll
-    for (int i = 0; i < STREAM_BATCH; i++) d[i] = src[w[i] & IDX_MASK];
+    for (int i = 0; i < STREAM_BATCH; i++)
+      if (i < n) d[i] = src[w[i] & IDX_MASK];
 #pragma unroll
     for (int i = 0; i < STREAM_BATCH; i++) {
-      // past the end of a stream of one batch, no term: its rounds are known when the kernel is built, so this folds
-      // away (a longer stream's last batch keeps its padding terms, which add zeros: a branch per word there costs
-      // more than the terms)
-      if (rounds <= STREAM_BATCH && q0 + i >= rounds) break;
+      if (i >= n) break;
       acc += term<DBL>(w[i], d[i]);
       const bool end = (w[i] >> END) & 1u;
       *(end ? dst + ((w[i] >> DEST) & DEST_MASK) : trash) = acc;
       acc = end ? 0.0 : acc;
     }
-  }
+  };
+  const int full = rounds / STREAM_BATCH * STREAM_BATCH;
+#pragma unroll 1
+  for (int q0 = 0; q0 < full; q0 += STREAM_BATCH) batch(q0, STREAM_BATCH);
+  if (rounds > full) batch(full, rounds - full);
 }
 
 // One Jacobian stream into the cell's LDS: entries of -Jn at their offsets.
 template <typename WP>
 __device__ __forceinline__ void jac_stream(WP __restrict__ words, int rounds, const double* DV, double* ws, int lane) {
-  stream_sum<T_IDX_MASK, T_DBL, T_END, T_DEST, T_DEST_MASK>(words, rounds, (u32)NDV, DV, ws, ws + L_TRASH + lane, lane);
+  stream_sum<T_IDX_MASK, T_DBL, T_END, T_DEST, T_DEST_MASK>(words, rounds, DV, ws, ws + L_TRASH + lane, lane);
 }
 
 // The stream words in the workgroup's LDS (WORDS_LDS: the call kernel copies them there at its start, after its
@@ -392,11 +395,11 @@ __device__ __forceinline__ double rhs(double Yl, double Yt, double& Fl, double& 
   wave_sync();
   double* part = ws + L_PART;
   if constexpr (LW)
-    stream_sum<SV_R_MASK, SV_DBL, SV_END, SV_PIECE, SV_PIECE_MASK>(lds_words(ws) + W_SV, SV_ROUNDS, (u32)NR, V, part,
+    stream_sum<SV_R_MASK, SV_DBL, SV_END, SV_PIECE, SV_PIECE_MASK>(lds_words(ws) + W_SV, SV_ROUNDS, V, part,
                                                                   ws + L_TRASH + lane, lane);
   else
-    stream_sum<SV_R_MASK, SV_DBL, SV_END, SV_PIECE, SV_PIECE_MASK>(svw, SV_ROUNDS, (u32)NR, V, part,
-                                                                  ws + L_TRASH + lane, lane);
+    stream_sum<SV_R_MASK, SV_DBL, SV_END, SV_PIECE, SV_PIECE_MASK>(svw, SV_ROUNDS, V, part, ws + L_TRASH + lane,
+                                                                  lane);
   wave_sync();
   if (lane < K0) {
     double s = 0.0;

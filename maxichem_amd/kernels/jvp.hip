@@ -196,8 +196,16 @@ __device__ __forceinline__ void rhs_tangent(double Yl, double Yt, double Yel, do
   double* V = ts + J_V;  // [4][NR + 1]
   constexpr int SV = NR + 1;
   auto nh = [&](int i) { return Hyper{n[i], n[SN + i], HD ? n[2 * SN + i] : 0.0, HD ? n[3 * SN + i] : 0.0}; };
-  for (int r = lane; r < NR; r += WAVE) {
-    const u64 w = rxe[r];
+  // every round's reaction word loaded first (in bounds: the last round's spare lanes read reaction NR - 1), so the
+  // rounds wait for one memory latency, not one each
+  u64 wr[RROUNDS];
+#pragma unroll
+  for (int q = 0; q < RROUNDS; q++) wr[q] = rxe[q * WAVE + lane < NR ? q * WAVE + lane : NR - 1];
+#pragma unroll
+  for (int q = 0; q < RROUNDS; q++) {
+    const int r = q * WAVE + lane;
+    if (r >= NR) break;
+    const u64 w = wr[q];
     const Hyper pf = hmul(hmul(nh(species_of(w, 0)), nh(species_of(w, 1))), nh(species_of(w, 2)));
     const Hyper pr = hmul(hmul(nh(species_of(w, 3)), nh(species_of(w, 4))), nh(species_of(w, 5)));
     const Hyper vf = hmul(Hyper{rk[r], rkd[r], 0.0, 0.0}, pf);
@@ -212,18 +220,28 @@ __device__ __forceinline__ void rhs_tangent(double Yl, double Yt, double Yel, do
   // S v by the rhs's lane streams, four parts at once
   double* part = ts + J_PART;  // [4][NPART]
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int q = 0; q < SV_ROUNDS; q++) {
-    const u32 w = svw[q * WAVE + lane];
-    const int r = (int)(w & SV_R_MASK);
-    const double c = coefficient(w, SV_NEG, SV_DBL);
+  // the words of TB rounds loaded together, ahead of their terms (one memory latency per batch; a batch of 16 would
+  // make the tangent kernel spill)
+  constexpr int TB = 8;
+#pragma unroll 1
+  for (int q0 = 0; q0 < SV_ROUNDS; q0 += TB) {
+    u32 wb[TB];
 #pragma unroll
-    for (int p = 0; p < 4; p++) acc[p] = fma(c, V[p * SV + r], acc[p]);
-    if ((w >> SV_END) & 1u) {
-      const int piece = (int)((w >> SV_PIECE) & SV_PIECE_MASK);
+    for (int i = 0; i < TB; i++) wb[i] = q0 + i < SV_ROUNDS ? svw[(q0 + i) * WAVE + lane] : (u32)NR;
 #pragma unroll
-      for (int p = 0; p < 4; p++) {
-        part[p * NPART + piece] = acc[p];
-        acc[p] = 0.0;
+    for (int i = 0; i < TB; i++) {
+      const u32 w = wb[i];
+      const int r = (int)(w & SV_R_MASK);
+      const double c = coefficient(w, SV_NEG, SV_DBL);
+#pragma unroll
+      for (int p = 0; p < 4; p++) acc[p] = fma(c, V[p * SV + r], acc[p]);
+      if ((w >> SV_END) & 1u) {
+        const int piece = (int)((w >> SV_PIECE) & SV_PIECE_MASK);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+          part[p * NPART + piece] = acc[p];
+          acc[p] = 0.0;
+        }
       }
     }
   }

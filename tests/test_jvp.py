@@ -18,6 +18,8 @@ against the call, tests/test_kernel_copies.py).
   vjp's cotangents of lm (per cell) and rc (summed over the cells) pass the dot-product test.
 - The recovery: cells the call leaves unfinished are recomputed with their tangent; results and flags are launch's
   bit for bit, and the tangent matches the oracle's whole-call differences through its recovery.
+- SNCHO (27 lead rows, the kernels' other lead-row paths): the primal bit for bit, the adaptive tangent against the
+  differences and the dot-product test.
 """
 
 import dataclasses
@@ -42,10 +44,10 @@ DOT_TOLERANCE = 1e-9
 DOT_TOLERANCE_ADAPTIVE = 1e-4
 
 
-def case(seed: int = 5):
-    """NCHO cells relaxed by the oracle (dt 1e8 s, dark), then called for dt 300 s, a quarter lit; random
-    directions: x moved relatively with its sum kept, T and P by up to 0.5 %, J by up to 50 %."""
-    net = standin.load("NCHO")
+def case(seed: int = 5, shape: str = "NCHO"):
+    """Cells of the network shape relaxed by the oracle (dt 1e8 s, dark), then called for dt 300 s, a quarter lit;
+    random directions: x moved relatively with its sum kept, T and P by up to 0.5 %, J by up to 50 %."""
+    net = standin.load(shape)
     tab = _tables.build(net, True)
     orc = oracle.Oracle(tab)
     inp = workloads.config1(net.species, len(net.photo), cells=CELLS, seed=seed)
@@ -199,9 +201,9 @@ ADAPT_EPS = (1e-6, 1e-5)  # two steps of the whole call's central differences al
 ADAPT_TOLERANCE = 1e-4  # of a cell's largest entry; the differences of the two steps agree to 1e-5 on the cells used
 
 
-def adaptive_case(seed: int = 13):
+def adaptive_case(seed: int = 13, shape: str = "NCHO"):
     """case() with directions of dt (up to 1 %) and of the first step h (up to 10 %) as well."""
-    net, orc, inp, d = case(seed)
+    net, orc, inp, d = case(seed, shape)
     rng = np.random.default_rng(seed + 1)
     d["ddt"] = 0.01 * inp["dt"] * rng.uniform(-1, 1, CELLS)
     d["dh"] = 0.1 * inp["h"] * rng.uniform(-1, 1, CELLS)
@@ -536,3 +538,45 @@ def test_tangent_through_the_recovery(gpu):
           f"{errh[use_h & rec].max():.2e}")  # fmt: skip
     assert (use & rec).sum() >= CELLS // 4 and (use_h & rec).sum() >= CELLS // 8
     assert err[use].max() <= RATE_TOLERANCE and errh[use_h].max() <= RATE_TOLERANCE
+
+
+@pytest.mark.gpu
+def test_tangent_and_adjoint_with_many_lead_rows(gpu):
+    # SNCHO's 27 lead rows take the kernels' other paths (the lead LU's rolled updates, the lead rows' N_j and solves
+    # by v_readlane): the primal is launch's bit for bit, the adaptive tangent matches the oracle's whole-call
+    # differences and vjp is its transpose, as on NCHO
+    from maxichem_amd.kincuda_ad import ADSolver
+
+    net, orc, inp, d = adaptive_case(seed=21, shape="SNCHO")
+    base, keep, fd = adaptive_differences(orc, inp, d)
+    (x1, h1), (x2, h2) = fd
+    args = [t(inp[k]) for k in ("x", "T", "P", "dt", "h")]
+    tangents = [t(d[k]) for k in ("dx", "dT", "dP", "ddt", "dh")]
+    with plain_solver(net) as plain:
+        want = plain.launch(*args, J=t(inp["J"]))
+    rng = np.random.default_rng(23)
+    gx, gh = rng.standard_normal((CELLS, len(net.species))), rng.standard_normal(CELLS) / inp["h"]
+    with ADSolver(net, "cuda", photo=True) as ad:
+        r = ad.jvp(*args, *tangents, J=t(inp["J"]), dJ=t(d["dJ"]))
+        tries = ad.tries.cpu().numpy()
+        v = ad.vjp(*args, t(gx), gh=t(gh), J=t(inp["J"]), rates=False)
+    for name in ("x", "h", "steps"):
+        assert torch.equal(getattr(r, name), getattr(want, name)), name
+        assert torch.equal(getattr(v, name), getattr(want, name)), name
+    dxg, dhg = r.dx.cpu().numpy(), r.dh.cpu().numpy()
+    scale = np.abs(x1).max(1)
+    use = keep & (np.abs(x1 - x2).max(1) <= 1e-5 * scale) & (tries == base["tries"])
+    use &= want.steps.cpu().numpy() == base["steps"]
+    err = np.abs(dxg - x1).max(1)[use] / scale[use]
+    g = v.cotangents
+    left = np.concatenate([gx * dxg, (gh * dhg)[:, None]], axis=1)
+    terms = np.concatenate([g.x.cpu().numpy() * d["dx"], (g.T.cpu().numpy() * d["dT"])[:, None],
+                            (g.P.cpu().numpy() * d["dP"])[:, None], g.J.cpu().numpy() * d["dJ"],
+                            (g.dt.cpu().numpy() * d["ddt"])[:, None], (g.h.cpu().numpy() * d["dh"])[:, None]],
+                           axis=1)  # fmt: skip
+    ok = np.isfinite(left.sum(1)) & np.isfinite(terms.sum(1))
+    dot = np.abs(left.sum(1) - terms.sum(1))[ok] / (np.abs(terms).sum(1) + np.abs(left).sum(1))[ok]
+    print(f"SNCHO: {int(use.sum())} of {CELLS} cells compared, largest tangent error of x {err.max():.2e}; "
+          f"dot-product mismatch {dot.max():.2e} on {int(ok.sum())} cells")  # fmt: skip
+    assert use.sum() >= CELLS // 3 and err.max() <= ADAPT_TOLERANCE
+    assert ok.sum() >= CELLS * 3 // 4 and dot.max() <= DOT_TOLERANCE_ADAPTIVE

@@ -249,3 +249,76 @@ def test_capped_coefficients_match_closed_forms(read_network, P):
         q = equations.index(eq)
         assert kf[q] == pytest.approx(want_f, rel=1e-13, abs=0.0), eq
         assert kr[q] == pytest.approx(want_r, rel=1e-13, abs=0.0), eq
+
+
+# the tangents of the capped coefficients (the tangent kernel's rates_tangent) against the closed forms' derivatives:
+# d ln k / dT at fixed P and d ln k / d ln P, from the same closed forms as above (a capped forward direction K_COLL nd
+# has -1/T and 1; the other follows by the NASA-9 ratio's d/dT; a direction capped at K_COLL has 0)
+DERIVATIVE_TOLERANCE = 1e-11  # of 1/T along T, of 1 along ln P: exp and log rounding
+
+
+def capped_log_derivatives(net, P):
+    """{equation: (d ln kf/dT, d ln kr/dT, d ln kf/d ln P, d ln kr/d ln P)} at T_COLL and P, fixed P along T."""
+    t = T_COLL
+
+    def dg(q):  # d(G/RT)/dT = -(H/RT)/T of a constant-Cp species: H/RT = cp + h / T
+        cp, h = net.lo[net.species.index(q)][[2, 7]]
+        return -(cp + h / t) / t
+
+    d_diss = 2 * dg("H") - dg("H2") + 1 / t  # d ln(kr / kf)/dT of H2 + M -> H + H + M
+    d_assoc = dg("O2") - 2 * dg("O") - 1 / t  # of O + O + M -> O2 + M
+    if P == P_LOW:  # the forward directions capped at K_COLL nd: -1/T along T, 1 along ln P
+        h2, o2 = (-1 / t, d_diss - 1 / t, 1.0, 1.0), (-1 / t - d_assoc, -1 / t, 1.0, 1.0)
+    else:  # the reverse of H2 + M and the forward of O + O + M capped at K_COLL
+        h2, o2 = (-d_diss, 0.0, 0.0, 0.0), (0.0, d_assoc, 0.0, 0.0)
+    return {"H2 + M -> H + H + M": h2, "O + O + M -> O2 + M": o2, "N_1 + M -> N_2 + M": (-1 / t, -1 / t, 1.0, 1.0),
+            "N_3 -> N_4": (0.0, 0.0, 0.0, 0.0)}  # fmt: skip
+
+
+@pytest.mark.gpu
+def test_capped_coefficients_tangents_match_closed_forms(read_network, gpu):
+    import torch
+
+    from maxichem_amd import _hip, _kernels, _tables
+
+    net = read_network("synthetic_collision")
+    tab = _tables.build(net, photo=True)
+    probe = (Path(__file__).resolve().parent / "rate_tangent_probe.hip").read_text()
+    src = _kernels.source(tab, 1, 1e-6, extra=("jvp.hip",)) + "\n" + probe
+    module = _hip.Module(_hip.compile_source(src, "rate_tangent_probe.hip"), torch.cuda.current_device())
+    try:
+        fn = module.function("mxc_rate_tangent_probe")
+        host = _kernels.host_tables(tab)
+        tables = [torch.from_numpy(host[k]).cuda() for k in _kernels.NET_FIELDS]
+        netp = torch.tensor([a.data_ptr() for a in tables], dtype=torch.int64, device="cuda")
+        # four cells: (P_LOW, P_HIGH) x (along T at fixed P, along ln P)
+        cells = [(P, along) for P in (P_LOW, P_HIGH) for along in (0, 1)]
+        T = torch.full((4,), T_COLL, dtype=torch.float64, device="cuda")
+        P = torch.tensor([c[0] for c in cells], dtype=torch.float64, device="cuda")
+        nd = P / (helpers.BOLTZMANN * T_COLL)
+        Td = torch.tensor([1.0 if a == 0 else 0.0 for _, a in cells], dtype=torch.float64, device="cuda")
+        ndd = torch.where(Td > 0, -nd / T_COLL, nd)  # d nd along T at fixed P, or along ln P
+        J = torch.full((4, tab.nj), J_COLL, dtype=torch.float64, device="cuda")
+        out = [torch.full((4, tab.nr), float("nan"), dtype=torch.float64, device="cuda") for _ in range(4)]
+        p = _hip.ptr
+        lds = 8 * (tab.layout()["SIZE"] + _kernels._jvp_region(tab))
+        _hip.launch(fn, 4, 64, lds, torch.cuda.current_stream().cuda_stream,
+                    [p(netp), p(T), p(P), p(Td), p(ndd), p(J), *[p(o) for o in out]])  # fmt: skip
+        torch.cuda.synchronize()
+        kf, kr, dkf, dkr = (o.cpu().numpy() for o in out)
+    finally:
+        module.unload()
+    equations = [r["eq"] for r in net.reac] + [r["eq"] for r in net.photo]
+    worst = 0.0
+    for i, (Pc, along) in enumerate(cells):
+        scale = 1.0 / T_COLL if along == 0 else 1.0
+        for eq, (want_f, want_r) in capped_pairs(net, Pc).items():
+            q = equations.index(eq)
+            assert kf[i, q] == pytest.approx(want_f, rel=1e-13, abs=0.0), eq
+            assert kr[i, q] == pytest.approx(want_r, rel=1e-13, abs=0.0), eq
+        for eq, d in capped_log_derivatives(net, Pc).items():
+            q = equations.index(eq)
+            for value, want in ((dkf[i, q] / kf[i, q], d[2 * along]), (dkr[i, q] / kr[i, q], d[2 * along + 1])):
+                worst = max(worst, abs(value - want) / scale)
+                assert abs(value - want) <= DERIVATIVE_TOLERANCE * scale, (eq, Pc, along, value, want)
+    print(f"capped coefficients' tangents against the closed forms: largest error {worst:.2e} (of 1/T or 1)")

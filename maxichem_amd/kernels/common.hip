@@ -185,8 +185,10 @@ using NetP = const __attribute__((address_space(4))) Net*;
 
 // ---- rate coefficients (the reference's law, collision_limit, gibbs_rt and rate_pair: common.cuh:135-242) ----
 
-__device__ __forceinline__ double law(gptr<double> __restrict__ q, int fl, double T, double lT, double iT, double nd) {
-  double k = exp(q[0] + q[1] * lT - q[2] * iT);
+// a0, a1, a2: q[0..2], the first Arrhenius term's, which the caller may have loaded ahead
+__device__ __forceinline__ double law(gptr<double> __restrict__ q, int fl, double T, double lT, double iT, double nd,
+                                      double a0, double a1, double a2) {
+  double k = exp(a0 + a1 * lT - a2 * iT);
   if (fl & FL_SECOND_K0) k += exp(q[3] + q[4] * lT - q[5] * iT);
   if (fl & FL_FALLOFF) {
     const double pr = k * nd / exp(q[6] + q[7] * lT - q[8] * iT);
@@ -206,6 +208,9 @@ __device__ __forceinline__ double law(gptr<double> __restrict__ q, int fl, doubl
   if (fl & FL_EFF) k *= q[13];
 #endif
   return k;
+}
+__device__ __forceinline__ double law(gptr<double> __restrict__ q, int fl, double T, double lT, double iT, double nd) {
+  return law(q, fl, T, lT, iT, nd, q[0], q[1], q[2]);
 }
 
 // A direction's bimolecular-equivalent rate: M-folded k times nd^(molecules - 2); one molecule without M: none.
@@ -245,16 +250,30 @@ __device__ __forceinline__ void cell_rates(double* ws, double* rk, double T, dou
   if (lane == 0) g[NS] = 0.0;
   wave_sync();
   const gptr<int> rord = net->rord;
+  // the rounds' table words are loaded a round ahead (the reaction's index two rounds ahead), so that the chain of
+  // dependent loads rord -> rf, rxe, rc is not waited on in each round; a lane past NR reads reaction rord[NR - 1]
+  auto at = [](int q) { return q < NR ? q : NR - 1; };
+  int rn = rord[at(lane)], rnn = rord[at(lane + WAVE)];
+  int fn = rf[rn];
+  u64 wn = rxe[rn];
+  double an0 = rc[RCW * rn], an1 = rc[RCW * rn + 1], an2 = rc[RCW * rn + 2];
 #pragma unroll 1
   for (int q = lane; q < NR; q += WAVE) {
-    const int r = rord[q];  // a round's reactions mostly of one law, so it runs the code of few laws
-    const int fl = rf[r];
+    const int r = rn, fl = fn;  // a round's reactions mostly of one law, so it runs the code of few laws
+    const u64 w = wn;
+    const double a0 = an0, a1 = an1, a2 = an2;
+    rn = rnn;
+    rnn = rord[at(q + 2 * WAVE)];
+    fn = rf[rn];
+    wn = rxe[rn];
+    an0 = rc[RCW * rn];
+    an1 = rc[RCW * rn + 1];
+    an2 = rc[RCW * rn + 2];
     double f = 0.0, b = 0.0;
     if (fl & FL_PHOTO) {
       f = Jc[r - NRT];
     } else {
-      const u64 w = rxe[r];
-      const double k = law(rc + RCW * r, fl, T, lT, iT, nd);
+      const double k = law(rc + RCW * r, fl, T, lT, iT, nd, a0, a1, a2);
       const double dg = g[species_of(w, 3)] + g[species_of(w, 4)] + g[species_of(w, 5)] - g[species_of(w, 0)] -
                         g[species_of(w, 1)] - g[species_of(w, 2)];
       const int mf = (fl >> FL_NF_SHIFT) & 3, mr = (fl >> FL_NR_SHIFT) & 3;

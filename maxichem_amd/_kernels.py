@@ -124,10 +124,10 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixe
         "TMIN": C.T_MIN, "TMAX": C.T_MAX, "XSUMTOL": C.X_SUM_TOL, "DRIFT": float(drift),
         "ETOT": float(tab.atoms.sum()), "KCOLL": C.K_COLL, "KBOLTZ": C.KB, "P0BAR": C.P0,
     }  # fmt: skip
-    # the tangent kernel's region per cell after the call's (kernels/jvp.hip): rate tangents [2 NR], then the
-    # hyper-dual n [4][NS + 1] (G / RT and its tangent there while the rates are computed), v [4][NR + 1], pieces
-    # [4][NPART]
-    jn = 2 * tab.nr
+    # the tangent kernel's region per cell after the call's (kernels/jvp.hip): rate tangents [2 NR] (in the
+    # wavefront's global slot after its rates instead, with the rates in global scratch), then the hyper-dual n
+    # [4][NS + 1] (G / RT and its tangent there while the rates are computed), v [4][NR + 1], pieces [4][NPART]
+    jn = 0 if rates_global else 2 * tab.nr
     jv = jn + 4 * (tab.ns + 1)
     jp = jv + 4 * (tab.nr + 1)
     jsize = jp + 4 * tab.npart
@@ -157,20 +157,27 @@ def header(tab: Tables, cpw: int, drift: float, batch: int = DEFAULT_BATCH, mixe
     return "\n".join(lines) + "\n"
 
 
-def _jvp_region(tab: Tables) -> int:
-    jsize = 2 * tab.nr + 4 * (tab.ns + 1) + 4 * (tab.nr + 1) + 4 * tab.npart
+def _jvp_region(tab: Tables, rates_global: bool = False) -> int:
+    jsize = (0 if rates_global else 2 * tab.nr) + 4 * (tab.ns + 1) + 4 * (tab.nr + 1) + 4 * tab.npart
     return jsize + (jsize & 1)
 
 
-def jvp_lds_bytes(tab: Tables, cpw: int) -> int:
-    """Dynamic LDS of a tangent workgroup (kernels/jvp.hip): each cell's call region and its tangent region."""
-    return 8 * (tab.layout()["SIZE"] + _jvp_region(tab)) * cpw
+def jvp_lds_bytes(tab: Tables, cpw: int, rates_global: bool = False) -> int:
+    """Dynamic LDS of a tangent workgroup (kernels/jvp.hip): each cell's call region and its tangent region (the
+    rates and their tangents in global scratch when rates_global)."""
+    return 8 * (tab.layout(False, not rates_global)["SIZE"] + _jvp_region(tab, rates_global)) * cpw
 
 
-def jvp_recover_lds_bytes(tab: Tables) -> int:
+def jvp_recover_lds_bytes(tab: Tables, rates_global: bool = False) -> int:
     """Dynamic LDS of a tangent recovery workgroup (mxc_jvp_recover): the recovery's, with the tangent region after
     the cell's call region."""
-    return lds_bytes(tab, 1)[1] + 8 * _jvp_region(tab)
+    return lds_bytes(tab, 1, rates_global=rates_global)[1] + 8 * _jvp_region(tab, rates_global)
+
+
+# doubles of a tangent wavefront's (or tangent recovery workgroup's) global slot with the rates in global scratch:
+# the rates kf, kr, then their tangents
+def jvp_slot(tab: Tables) -> int:
+    return 4 * tab.nr
 
 
 def lds_bytes(tab: Tables, cpw: int, mixed: bool = False, rates_global: bool = False, wps: int = 1,

@@ -488,13 +488,18 @@ extern "C" __global__ void __launch_bounds__(WAVE* CPW)
             const double* __restrict__ x0d, const double* __restrict__ Td, const double* __restrict__ Pd,
             const double* __restrict__ Jd, double* __restrict__ xd, int adapt, const double* __restrict__ dtd,
             const double* __restrict__ hd, double* __restrict__ hdo, const double* __restrict__ lm,
-            const double* __restrict__ lmd, const double* __restrict__ rcd, int D) {
+            const double* __restrict__ lmd, const double* __restrict__ rcd, int D, double* __restrict__ rsc) {
   extern __shared__ double lds[];
   const int lane = (int)(threadIdx.x & (WAVE - 1)), wv = (int)(threadIdx.x / WAVE);
   double* ws = lds + wv * (L_SIZE + J_SIZE);
   double* ts = ws + L_SIZE;
+#if RATES_GLOBAL
+  double* rk = rsc + (size_t)((int)blockIdx.x * CPW + wv) * (4 * NR);  // this wavefront's slot: the rates,
+  double* rkd = rk + 2 * NR;                                            // then their tangents
+#else
   double* rk = ws + L_KF;
   double* rkd = ts + J_KD;
+#endif
   const int waves = (int)gridDim.x * CPW;
   int q = (int)blockIdx.x * CPW + wv;
   double c[NT];
@@ -696,15 +701,21 @@ extern "C" __global__ void __launch_bounds__(WAVE* REC_WAVES)
                     double atol, int max_steps, const double* __restrict__ lm, const double* __restrict__ x0d,
                     const double* __restrict__ Td, const double* __restrict__ Pd, const double* __restrict__ Jd,
                     double* __restrict__ xd, int adapt, const double* __restrict__ dtd, const double* __restrict__ hd,
-                    double* __restrict__ hdo, const double* __restrict__ lmd, const double* __restrict__ rcd) {
+                    double* __restrict__ hdo, const double* __restrict__ lmd, const double* __restrict__ rcd,
+                    double* __restrict__ rsc) {
   const int nfl = count[COUNT_RECOMPUTED];
   if ((int)blockIdx.x >= nfl) return;
   extern __shared__ double lds[];
   const int lane = (int)(threadIdx.x & (WAVE - 1)), wv = (int)(threadIdx.x / WAVE);
   double* ws = lds;
   double* ts = ws + L_SIZE;
+#if RATES_GLOBAL
+  double* rk = rsc + (size_t)blockIdx.x * (4 * NR);  // this workgroup's slot: the rates, then their tangents
+  double* rkd = rk + 2 * NR;
+#else
   double* rk = ws + L_KF;
   double* rkd = ts + J_KD;
+#endif
   double* W = ts + J_SIZE;
   double* dinv = W + NS * NS;
   double* yv = dinv + NS;

@@ -88,11 +88,13 @@ class ADSolver(CudaSolver):
             raise ValueError(f"cap: an integer expected, got {cap!r}")
         if cap < 1:
             raise ValueError(f"cap: at least 1 stored attempt expected, got {cap}")
-        super().__init__(net, device, drift=drift, photo=photo, variant=(1, _kernels.DEFAULT_BATCH, 1, 0))
+        # the rates in global scratch: a tangent cell then needs 40 KB of LDS instead of 54 (NCHO), four cells per
+        # compute unit instead of three (every SIMD busy); the call kernel of this module gives the same bits
+        super().__init__(net, device, drift=drift, photo=photo, variant=(1, _kernels.DEFAULT_BATCH, 1, 1))
         self.cap = cap
         with torch.cuda.device(self.dev):
             self._jvp = self._module.function("mxc_jvp")
-            self.lds_jvp = _kernels.jvp_lds_bytes(self.tab, 1)
+            self.lds_jvp = _kernels.jvp_lds_bytes(self.tab, 1, True)
             info = _hip.device_info(self.dev.index)
             if self.lds_jvp > info["lds_per_block"]:
                 raise ValueError(f"one cell's tangent kernel needs {self.lds_jvp} B of LDS; the device has "
@@ -100,15 +102,23 @@ class ADSolver(CudaSolver):
             self.max_blocks_jvp = info["compute_units"] * max(1, _hip.occupancy(self._jvp, WAVE, self.lds_jvp))
             self._rc_cotangent = self._module.function("mxc_rc_cotangent")
             self._jvp_recover = self._module.function("mxc_jvp_recover")
-            self.lds_jvp_rec = _kernels.jvp_recover_lds_bytes(self.tab)
+            self.lds_jvp_rec = _kernels.jvp_recover_lds_bytes(self.tab, True)
             if self.lds_jvp_rec > info["lds_per_block"]:
                 raise ValueError(f"one cell's tangent recovery needs {self.lds_jvp_rec} B of LDS; the device has "
                                  f"{info['lds_per_block']}")  # fmt: skip
             self.rec_blocks_jvp = info["compute_units"] * max(
                 1, _hip.occupancy(self._jvp_recover, WAVE * _kernels.REC_WAVES, self.lds_jvp_rec))
+            # each tangent wavefront's and tangent recovery workgroup's slot of rates and rate tangents
+            self._jrsc = torch.empty(max(self.max_blocks_jvp, self.rec_blocks_jvp) * _kernels.jvp_slot(self.tab),
+                                     dtype=torch.float64, device=self.dev)  # fmt: skip
         # the rate table's width as the reference gives it: the nine Arrhenius columns, and the Troe parameters and
         # the efficiency when a reaction has them
         self.rcw = self.tab.rc.shape[1] if self.tab.has_troe or self.tab.has_eff else ARRHENIUS_COLUMNS
+
+    def close(self) -> None:
+        """As CudaSolver.close, and drop the tangent kernels' slots of rates."""
+        super().close()
+        self._jrsc = None
 
     @property
     def rc0(self) -> torch.Tensor:
@@ -206,7 +216,7 @@ class ADSolver(CudaSolver):
         args = [p(net), p(x), p(T), p(P), p(dt), p(h), p(ho), p(xo), p(done), p(nok), p(status), p(redo), p(lst),
                 p(self.count), p(self.tries), p(jx), ctypes.c_int(B), ctypes.c_double(rtol), ctypes.c_double(atol),
                 ctypes.c_int(max_steps), p(dx), p(dT), p(dP), p(dJ), p(xd), ctypes.c_int(int(bool(adapt))), p(ddt),
-                p(dh), p(hd), p(lm), p(dlm), p(drc), ctypes.c_int(D)]  # fmt: skip
+                p(dh), p(hd), p(lm), p(dlm), p(drc), ctypes.c_int(D), p(self._jrsc)]  # fmt: skip
         if B:
             _hip.launch(self._jvp, max(1, min(B * D, self.max_blocks_jvp)), WAVE, self.lds_jvp, stream, args)
         # the cells the guard failed: the recovery's result with each direction's tangent. Direction 0's launch keeps
@@ -223,7 +233,7 @@ class ADSolver(CudaSolver):
                         p(self.tries if d == 0 else None), p(jx), ctypes.c_double(rtol), ctypes.c_double(atol),
                         ctypes.c_int(REC_STEPS * max_steps), p(lm), p(dx[d]), p(row(dT, d)), p(row(dP, d)),
                         p(row(dJ, d)), p(xd[d]), ctypes.c_int(int(bool(adapt))), p(row(ddt, d)), p(row(dh, d)),
-                        p(hd[d]), p(row(dlm, d)), p(row(drc, d))]  # fmt: skip
+                        p(hd[d]), p(row(dlm, d)), p(row(drc, d)), p(self._jrsc)]  # fmt: skip
             _hip.launch(self._jvp_recover, max(1, min(B, self.rec_blocks_jvp)), WAVE * _kernels.REC_WAVES,
                         self.lds_jvp_rec, stream, rec_args)  # fmt: skip
         self._status = status

@@ -64,7 +64,8 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--train-pairs", type=int, default=2_000_000,
                     help="plain calls the learned workload's surrogate is trained on")
     ap.add_argument("--oracle-sample", type=int, default=256, help="cells of the learned workload the oracle checks")
-    ap.add_argument("--workload", default="calls", choices=("calls", "uneven", "coupling", "learned"),
+    ap.add_argument("--directions", type=int, default=8, help="tangent directions per jvp (--workload tangent)")
+    ap.add_argument("--workload", default="calls", choices=("calls", "uneven", "coupling", "learned", "tangent"),
                     help="calls: the 65,536-cell call sequence (the metric's); uneven: 262,144 cells, one call each "
                          "from cold, half lit, dt log-uniform 1e1-1e6 s, by graph replay with the tuned variant; "
                          "coupling: the GCM-side conversion kernels on 65,536 cells, against HBM bandwidth; learned: "
@@ -500,6 +501,72 @@ def run_coupling(args, net, dev) -> int:
     return 0
 
 
+def run_tangent(args, net, dev) -> int:
+    """ADSolver.jvp with --directions random state directions per call (adaptive contract, the reference's default),
+    timed by HIP events around each jvp on its stream (the tangent kernel, the recovery of listed cells and the count
+    read), on the bench's cells after the dark relaxation and one lit call; beside it the plain call on the same
+    inputs, and for at most 8,192 cells the vjp by columns without the rate cotangents."""
+    import numpy as np
+    import torch
+
+    from maxichem_amd import workloads
+    from maxichem_amd.kincuda import CudaSolver
+    from maxichem_amd.kincuda_ad import ADSolver
+
+    B, D = args.cells, args.directions
+    d = workloads.config1(net.species, len(net.photo), B)
+
+    def g(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+    def events(call, n):
+        ms = []
+        for _ in range(n):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            call()
+            b.record()
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        return float(np.mean(ms))
+
+    T, P, dt, J = g(d["T"]), g(d["P"]), g(d["dt"]), g(d["J"])
+    with CudaSolver(net, dev, photo=True) as s:
+        r = s(g(d["x"]), T, P, g(d["dt_relax"]), g(d["h"]), J=torch.zeros_like(J))
+        r = s(r.x, T, P, dt, r.h, J=J)
+        x, h = r.x, r.h
+        call_ms = events(lambda: s(x, T, P, dt, h, J=J), max(1, args.steps))
+    rng = np.random.default_rng(0)
+    dx = g(rng.standard_normal((D, B, len(net.species))) * 1e-3) * x
+    z = torch.zeros(D, B, dtype=torch.float64, device=dev)
+    with ADSolver(net, dev, photo=True) as ad:
+        log(f"tangent solver: {ad.info}; tangent kernel LDS {ad.lds_jvp} B per cell, {ad.max_blocks_jvp} cells at once")
+
+        def jvp():
+            ad.jvp(x, T, P, dt, h, dx, z, z, z, z, J=J)
+
+        for _ in range(max(1, args.warmup)):
+            jvp()
+        jvp_ms = events(jvp, max(1, args.steps))
+        vjp_ms = None
+        if B <= 8192:
+            gx = g(rng.standard_normal((B, len(net.species))))
+            vjp_ms = events(lambda: ad.vjp(x, T, P, dt, h, gx, J=J, rates=False), 1)
+    log(f"{B} cells: call {call_ms:.1f} ms, jvp with {D} directions {jvp_ms:.1f} ms ({jvp_ms / D:.1f} ms a "
+        f"direction)" + ("" if vjp_ms is None else f", vjp without the rate cotangents {vjp_ms:.1f} ms"))  # fmt: skip
+    line = {
+        "metric": f"cell-directions/s of the call's tangent ({args.network}, {B} cells, {D} directions)",
+        "value": B * D / (jvp_ms * 1e-3), "unit": "cell-directions/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": jvp_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"tangent: {args.network} {B} cells after the dark relaxation and one lit call of "
+                               f"dt = 300 s, jvp along {D} random state directions (adapt=True)"},
+        "call_ms": call_ms, "jvp_ms": jvp_ms, "vjp_ms": vjp_ms, "roofline": None, "cpu_baseline": None,
+    }  # fmt: skip
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -522,6 +589,8 @@ def main() -> int:
         return run_coupling(args, net, dev)
     if args.workload == "learned":
         return run_learned(args, net, dev)
+    if args.workload == "tangent":
+        return run_tangent(args, net, dev)
     d = workloads.config1(net.species, nj, args.cells)
     log(f"network {args.network}: {len(net.species)} species, {len(net.reac)} thermal, {nj} photolysis; "
         f"{args.cells} cells")  # fmt: skip
